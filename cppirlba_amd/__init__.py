@@ -1,0 +1,431 @@
+"""MI355X-native IRLBA: Python mirror of the irlba:: API over the HIP engine.
+
+Names follow the reference (include/irlba/*.hpp):
+  Options            <- Options.hpp:19-83
+  compute()/Results  <- compute.hpp:141-308, 357-410
+  compute_simple()   <- compute.hpp:333-350, 426-433
+  pca()/PcaResults   <- pca.hpp:55-204
+  HipSparseMatrix    <- ParallelSparseMatrix (Matrix/sparse.hpp:438-589)
+  HipDenseMatrix     <- SimpleMatrix (Matrix/simple.hpp:124-178)
+  CenteredMatrix     <- Matrix/centered.hpp:139-204
+  ScaledMatrix       <- Matrix/scaled.hpp:195-268
+  Matrix             <- Matrix/interface.hpp:141-223 (caller-defined products)
+
+Every product, Gram-Schmidt step and restart rotation runs on the GPU
+through libirlba_hip.so; the package has no CPU fallback.
+"""
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import BF16, F32, F64, HipError, check, lib
+
+__all__ = [
+    "Options", "Results", "PcaResults", "Context", "Matrix", "HipSparseMatrix", "HipDenseMatrix", "CenteredMatrix",
+    "ScaledMatrix", "compute", "compute_simple", "pca", "column_stats", "generate_sparse", "default_context",
+    "HipError", "F64", "F32", "BF16",
+]
+
+_TYPES = {"f64": F64, "float64": F64, "fp64": F64, "f32": F32, "float32": F32, "fp32": F32, "bf16": BF16,
+          "bfloat16": BF16}
+
+
+def _type_code(t):
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return _TYPES[str(t).lower()]
+
+
+@dataclass
+class Options:
+    """irlba::Options (Options.hpp:27-82), same names and defaults."""
+    invariant_subspace_tolerance: float = -1.0
+    convergence_tolerance: float = 1e-5
+    singular_value_ratio_tolerance: float = -1.0
+    extra_work: int = 7
+    max_iterations: int = 1000
+    exact_for_small_matrix: bool = True
+    exact_for_large_number: bool = True
+    cap_number: bool = False
+    seed: int = 5489
+    initial: Optional[np.ndarray] = None
+
+    def _to_c(self):
+        o = _lib.Options()
+        o.invariant_subspace_tolerance = self.invariant_subspace_tolerance
+        o.convergence_tolerance = self.convergence_tolerance
+        o.singular_value_ratio_tolerance = self.singular_value_ratio_tolerance
+        o.extra_work = self.extra_work
+        o.max_iterations = self.max_iterations
+        o.exact_for_small_matrix = int(self.exact_for_small_matrix)
+        o.exact_for_large_number = int(self.exact_for_large_number)
+        o.cap_number = int(self.cap_number)
+        o.seed = self.seed
+        keep = None
+        if self.initial is not None:
+            keep = np.ascontiguousarray(self.initial, dtype=np.float64)
+            o.initial = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            o.initial_length = keep.size
+        return o, keep
+
+
+@dataclass
+class Results:
+    """compute.hpp:357-388."""
+    U: np.ndarray
+    V: np.ndarray
+    D: np.ndarray
+    iterations: int
+    converged: bool
+
+
+@dataclass
+class PcaResults:
+    """pca.hpp:146-179."""
+    scores: np.ndarray
+    rotation: np.ndarray
+    variances: np.ndarray
+    iterations: int
+    converged: bool
+
+
+class Context:
+    """One GPU and one HIP stream."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().irlba_hip_context_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            lib().irlba_hip_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(lib().irlba_hip_context_synchronize(self.handle))
+
+    def stats(self):
+        s = _lib.Stats()
+        check(lib().irlba_hip_get_stats(self.handle, ctypes.byref(s)))
+        return {name: getattr(s, name) for name, _ in _lib.Stats._fields_}
+
+    def set_profiling(self, enabled):
+        check(lib().irlba_hip_context_set_profiling(self.handle, int(enabled)))
+
+    @property
+    def stream(self):
+        return lib().irlba_hip_context_stream(self.handle)
+
+
+_default = None
+
+
+def default_context():
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+def _ptr(a):
+    """(pointer, is_device, keepalive) for a numpy array or a torch tensor."""
+    if hasattr(a, "data_ptr") and hasattr(a, "is_cuda"):
+        if a.is_cuda:
+            import torch
+
+            torch.cuda.synchronize(a.device)  # the engine runs on its own stream
+            return ctypes.c_void_p(a.data_ptr()), True, a
+        a = a.numpy()
+    return ctypes.c_void_p(a.ctypes.data), False, a
+
+
+class _Handle:
+    """A device-resident matrix (opaque irlba_hip_matrix)."""
+
+    def __init__(self, ctx, handle, keep=()):
+        self.ctx = ctx
+        self.handle = handle
+        self._keep = list(keep)
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None) is not None and self.handle.value:
+                lib().irlba_hip_matrix_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def rows(self):
+        r = ctypes.c_int64()
+        check(lib().irlba_hip_matrix_dims(self.handle, ctypes.byref(r), None))
+        return r.value
+
+    def cols(self):
+        c = ctypes.c_int64()
+        check(lib().irlba_hip_matrix_dims(self.handle, None, ctypes.byref(c)))
+        return c.value
+
+    @property
+    def shape(self):
+        return self.rows(), self.cols()
+
+    def multiply(self, right, adjoint=False):
+        """Workspace::multiply / AdjointWorkspace::multiply (interface.hpp:49,87)."""
+        nr, nc = self.shape
+        right = np.ascontiguousarray(right, dtype=np.float64)
+        if right.size != (nr if adjoint else nc):
+            raise ValueError("vector length does not match the matrix")
+        out = np.empty(nc if adjoint else nr)
+        check(lib().irlba_hip_multiply(self.handle, int(adjoint), right.ctypes.data, out.ctypes.data, 0))
+        return out
+
+    def adjoint_multiply(self, right):
+        return self.multiply(right, adjoint=True)
+
+    def realize(self):
+        """RealizeWorkspace::realize_copy (interface.hpp:125-130)."""
+        nr, nc = self.shape
+        out = np.empty(nr * nc)
+        check(lib().irlba_hip_realize(self.handle, out.ctypes.data, 0))
+        return out.reshape(nc, nr).T.copy()
+
+
+class HipSparseMatrix(_Handle):
+    """Device drop-in for ParallelSparseMatrix(nrow, ncol, x, i, p, column_major, num_threads).
+
+    ``storage``: value type kept in HBM ("f64" or "f32"; default: the input's).
+    ``accumulate``: "f64" or "f32" (mixed precision); default "f64" for FP64
+    storage and "f32" for FP32 storage.
+    """
+
+    def __init__(self, nrow, ncol, x, i, p, column_major=True, storage=None, accumulate=None, ctx=None):
+        ctx = ctx or default_context()
+        xp, xdev, xk = _ptr(x if hasattr(x, "data_ptr") else np.ascontiguousarray(x))
+        dtype = str(getattr(xk, "dtype", "float64")).replace("torch.", "")
+        in_t = F32 if "float32" in dtype else F64
+        if not xdev and in_t == F64:
+            xk = np.ascontiguousarray(xk, dtype=np.float64)
+            xp = ctypes.c_void_p(xk.ctypes.data)
+        ip, idev, ik = _ptr(i if hasattr(i, "data_ptr") else np.ascontiguousarray(i, dtype=np.int32))
+        pp, pdev, pk = _ptr(p if hasattr(p, "data_ptr") else np.ascontiguousarray(p, dtype=np.int64))
+        if not (xdev == idev == pdev):
+            raise ValueError("values, indices and pointers must all be on the host or all on the device")
+        st = _type_code(storage) if storage is not None else in_t
+        acc = _type_code(accumulate) if accumulate is not None else (F64 if st == F64 else F32)
+        h = ctypes.c_void_p()
+        check(lib().irlba_hip_sparse_create(ctx.handle, nrow, ncol, in_t, xp, ip, pp, int(column_major), st,
+                                            int(acc == F64), int(xdev), ctypes.byref(h)))
+        super().__init__(ctx, h)
+
+
+class HipDenseMatrix(_Handle):
+    """Device drop-in for SimpleMatrix over a dense matrix (simple.hpp:124-178).
+
+    ``storage`` "f64" (FP64 products), "f32" or "bf16" (FP32 accumulation).
+    """
+
+    def __init__(self, array, storage="f64", ctx=None):
+        ctx = ctx or default_context()
+        if hasattr(array, "data_ptr") and getattr(array, "is_cuda", False):
+            nr, nc = array.shape
+            col_major = not array.is_contiguous()
+            if col_major and not array.t().is_contiguous():
+                array = array.contiguous()
+                col_major = False
+            dt = str(array.dtype)
+            in_t = F64 if "float64" in dt else (F32 if "float32" in dt else BF16)
+            ptr, dev, keep = ctypes.c_void_p(array.data_ptr()), 1, array
+        else:
+            a = np.asarray(array)
+            nr, nc = a.shape
+            if a.flags.f_contiguous and not a.flags.c_contiguous:
+                col_major = True
+            else:
+                a = np.ascontiguousarray(a)
+                col_major = False
+            if a.dtype == np.float32:
+                in_t = F32
+            else:
+                a = a.astype(np.float64, copy=False)
+                in_t = F64
+            ptr, dev, keep = ctypes.c_void_p(a.ctypes.data), 0, a
+        h = ctypes.c_void_p()
+        check(lib().irlba_hip_dense_create(ctx.handle, nr, nc, in_t, ptr, int(col_major), _type_code(storage), dev,
+                                           ctypes.byref(h)))
+        super().__init__(ctx, h)
+
+
+class CenteredMatrix(_Handle):
+    """CenteredMatrix(matrix, center) (centered.hpp:139-159): A - 1 c'."""
+
+    def __init__(self, matrix, center):
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        h = ctypes.c_void_p()
+        check(lib().irlba_hip_centered_create(matrix.handle, c.ctypes.data, 0, ctypes.byref(h)))
+        super().__init__(matrix.ctx, h, keep=[matrix])
+
+
+class ScaledMatrix(_Handle):
+    """ScaledMatrix(matrix, scale, column, divide) (scaled.hpp:195-223)."""
+
+    def __init__(self, matrix, scale, column=True, divide=True):
+        s = np.ascontiguousarray(scale, dtype=np.float64)
+        h = ctypes.c_void_p()
+        check(lib().irlba_hip_scaled_create(matrix.handle, s.ctypes.data, int(column), int(divide), 0,
+                                            ctypes.byref(h)))
+        super().__init__(matrix.ctx, h, keep=[matrix])
+
+
+class Matrix:
+    """Caller-defined matrix (interface.hpp:141-223): subclass and implement
+    rows(), cols(), multiply(x) and adjoint_multiply(x) on host vectors; the
+    rest of IRLBA runs on the GPU."""
+
+    def rows(self):
+        raise NotImplementedError
+
+    def cols(self):
+        raise NotImplementedError
+
+    def multiply(self, right):
+        raise NotImplementedError
+
+    def adjoint_multiply(self, right):
+        raise NotImplementedError
+
+
+def _run_compute(matrix, number, options, ctx=None):
+    options = options or Options()
+    o, keep = options._to_c()
+    n = max(int(number), 0)
+    nout = ctypes.c_int64()
+    conv = ctypes.c_int32()
+    it = ctypes.c_int32()
+    if isinstance(matrix, _Handle):
+        nr, nc = matrix.shape
+        U = np.zeros(nr * n)
+        V = np.zeros(nc * n)
+        D = np.zeros(max(n, 1))
+        check(lib().irlba_hip_compute(matrix.handle, number, ctypes.byref(o), U.ctypes.data, V.ctypes.data,
+                                      D.ctypes.data, ctypes.byref(nout), ctypes.byref(conv), ctypes.byref(it), 0))
+    elif isinstance(matrix, Matrix):
+        ctx = ctx or default_context()
+        nr, nc = matrix.rows(), matrix.cols()
+        err = []
+
+        def cb(user, adjoint, right, output):
+            try:
+                nin = nr if adjoint else nc
+                nout_ = nc if adjoint else nr
+                x = np.ctypeslib.as_array(right, shape=(nin,))
+                y = matrix.adjoint_multiply(x.copy()) if adjoint else matrix.multiply(x.copy())
+                np.ctypeslib.as_array(output, shape=(nout_,))[:] = y
+                return 0
+            except Exception as e:  # reported after the call
+                err.append(e)
+                return 1
+
+        fn = _lib.MATVEC_FN(cb)
+        U = np.zeros(nr * n)
+        V = np.zeros(nc * n)
+        D = np.zeros(max(n, 1))
+        rc = lib().irlba_hip_compute_callback(ctx.handle, nr, nc, fn, None, number, ctypes.byref(o), U.ctypes.data,
+                                              V.ctypes.data, D.ctypes.data, ctypes.byref(nout), ctypes.byref(conv),
+                                              ctypes.byref(it))
+        if err:
+            raise err[0]
+        check(rc)
+    else:
+        raise TypeError("compute() takes a device matrix handle or an irlba Matrix subclass")
+    k = nout.value
+    return Results(U[: nr * k].reshape(k, nr).T.copy(), V[: nc * k].reshape(k, nc).T.copy(), D[:k].copy(),
+                   it.value, bool(conv.value))
+
+
+def compute(matrix, number, options=None):
+    """compute(matrix, number, options) -> Results (compute.hpp:141-308, 403-410)."""
+    return _run_compute(matrix, number, options)
+
+
+def compute_simple(array, number, options=None, storage="f64", ctx=None):
+    """compute_simple(matrix, number, options) (compute.hpp:333-350, 426-433)."""
+    return _run_compute(HipDenseMatrix(array, storage=storage, ctx=ctx), number, options)
+
+
+def column_stats(matrix, center=True, scale=True):
+    """Column means and standard deviations used by pca() (pca.hpp:66-109)."""
+    nc = matrix.cols()
+    c = np.zeros(nc)
+    s = np.zeros(nc)
+    check(lib().irlba_hip_column_stats(matrix.handle, int(center), int(scale), c.ctypes.data, s.ctypes.data, 0))
+    return (c if center else None), (s if scale else None)
+
+
+def pca(matrix, center, scale, number, options=None, storage="f64", ctx=None):
+    """pca(matrix, center, scale, number, options) -> PcaResults (pca.hpp:55-139, 197-204).
+
+    ``matrix`` is a device handle, or a dense array (wrapped like SimpleMatrix)."""
+    if not isinstance(matrix, _Handle):
+        matrix = HipDenseMatrix(matrix, storage=storage, ctx=ctx)
+    options = options or Options()
+    o, keep = options._to_c()
+    nr, nc = matrix.shape
+    n = max(int(number), 0)
+    S = np.zeros(nr * n)
+    R = np.zeros(nc * n)
+    var = np.zeros(max(n, 1))
+    nout = ctypes.c_int64()
+    conv = ctypes.c_int32()
+    it = ctypes.c_int32()
+    check(lib().irlba_hip_pca(matrix.handle, int(center), int(scale), number, ctypes.byref(o), S.ctypes.data,
+                              R.ctypes.data, var.ctypes.data, ctypes.byref(nout), ctypes.byref(conv),
+                              ctypes.byref(it), 0))
+    k = nout.value
+    return PcaResults(S[: nr * k].reshape(k, nr).T.copy(), R[: nc * k].reshape(k, nc).T.copy(), var[:k].copy(),
+                      it.value, bool(conv.value))
+
+
+def generate_sparse(nrow, ncol, density, seed, storage="f32", accumulate=None, ctx=None):
+    """Synthetic CSC built on the device: Bernoulli(density) positions, values
+    log1p(NB(mu=2, phi=0.5)); identical to the CPU generator for the same seed."""
+    ctx = ctx or default_context()
+    st = _type_code(storage)
+    acc = _type_code(accumulate) if accumulate is not None else (F64 if st == F64 else F32)
+    h = ctypes.c_void_p()
+    check(lib().irlba_hip_generate_sparse(ctx.handle, nrow, ncol, density, seed, st, int(acc == F64),
+                                          ctypes.byref(h)))
+    return _Handle(ctx, h)
+
+
+def generate_csc_host(nrow, ncol, density, seed, value_type="f32", ctx=None):
+    """The device generator's CSC copied to host arrays (values, indices, pointers)."""
+    import torch
+
+    ctx = ctx or default_context()
+    vt = _type_code(value_type)
+    nnz = ctypes.c_int64()
+    check(lib().irlba_hip_generate_csc(ctx.handle, nrow, ncol, density, seed, vt, None, None, None,
+                                       ctypes.byref(nnz)))
+    dev = torch.device("cuda", ctx.device)
+    vals = torch.empty(nnz.value, dtype=torch.float32 if vt == F32 else torch.float64, device=dev)
+    idx = torch.empty(nnz.value, dtype=torch.int32, device=dev)
+    ptr = torch.empty(ncol + 1, dtype=torch.int64, device=dev)
+    check(lib().irlba_hip_generate_csc(ctx.handle, nrow, ncol, density, seed, vt, ctypes.c_void_p(vals.data_ptr()),
+                                       ctypes.c_void_p(idx.data_ptr()), ctypes.c_void_p(ptr.data_ptr()),
+                                       ctypes.byref(nnz)))
+    ctx.synchronize()
+    return vals.cpu().numpy(), idx.cpu().numpy(), ptr.cpu().numpy()

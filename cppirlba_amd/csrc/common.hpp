@@ -1,0 +1,186 @@
+// Shared definitions for the MI355X (gfx950) IRLBA engine: error handling,
+// wave64 reductions and the launcher declarations that the host engine
+// (engine.cpp) calls.  Every kernel is written for 64-wide wavefronts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace irlba_hip {
+
+typedef std::int64_t i64;
+
+#define IRLBA_HIP_CHECK(expr)                                                                         \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            throw std::runtime_error(std::string("HIP error in " #expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define IRLBA_HIP_LAUNCH_CHECK() IRLBA_HIP_CHECK(hipGetLastError())
+
+enum StorageType { ST_F64 = 0, ST_F32 = 1, ST_BF16 = 2 };
+
+constexpr int kWave = 64;
+
+// Native clang vectors (the nontemporal load builtins take these, not HIP's struct vectors).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------ device helpers
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Sum within aligned groups of `Width` lanes (Width a power of two <= 64).
+template <int Width, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+    for (int off = Width / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Block-wide sum of up to two doubles; every thread gets the totals.
+// `scratch` needs 2 * (blockDim.x / 64) doubles of LDS.
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* scratch) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) {
+        scratch[wid] = a;
+        scratch[nw + wid] = b;
+    }
+    __syncthreads();
+    double sa = 0, sb = 0;
+    for (int w = 0; w < nw; ++w) {
+        sa += scratch[w];
+        sb += scratch[nw + w];
+    }
+    a = sa;
+    b = sb;
+    __syncthreads();
+}
+
+__device__ __forceinline__ float to_float(float v) { return v; }
+__device__ __forceinline__ float to_float(__hip_bfloat16 v) { return __bfloat162float(v); }
+__device__ __forceinline__ double to_double_any(double v) { return v; }
+__device__ __forceinline__ double to_double_any(float v) { return v; }
+__device__ __forceinline__ double to_double_any(__hip_bfloat16 v) { return __bfloat162float(v); }
+
+// ------------------------------------------------------------------ fused affine terms
+// A device operator is  M = D_r A D_c - sum_k a_k b_k'  (the closed form of any
+// chain of CenteredMatrix / ScaledMatrix decorators).  For one product the
+// "minor" side is the input and the "major" side the output.
+constexpr int kMaxTerms = 2;
+
+struct Prologue {                  // applied to the input vector while it is staged
+    const double* x = nullptr;     // input (minor length)
+    const double* xscale = nullptr;  // D on the input side (nullable)
+    int n_terms = 0;
+    const double* tdot[kMaxTerms] = {nullptr, nullptr};  // gamma_k = tdot_k . x (nullptr = ones)
+};
+
+struct Epilogue {                  // out[m] = oscale[m]*acc - sum_k tvec_k[m]*gamma_k - (*coef)*src[m]
+    double* out = nullptr;
+    const double* oscale = nullptr;
+    int n_terms = 0;
+    const double* tvec[kMaxTerms] = {nullptr, nullptr};  // nullptr = ones
+    const double* axpy_src = nullptr;
+    const double* axpy_coef = nullptr;  // device scalar
+};
+
+__device__ __forceinline__ double apply_epilogue(const Epilogue& e, i64 m, double acc, const double* gamma) {
+    double o = e.oscale ? acc * e.oscale[m] : acc;
+    for (int k = 0; k < e.n_terms; ++k) o -= (e.tvec[k] ? e.tvec[k][m] : 1.0) * gamma[k];
+    if (e.axpy_src) o -= (*e.axpy_coef) * e.axpy_src[m];
+    return o;
+}
+
+// ------------------------------------------------------------------ tiled compressed lines
+// One orientation of a sparse matrix: `n_major` output lines, the minor (input)
+// dimension split into `n_tiles` tiles of `tile_len` entries so that one tile of
+// the input vector fits in LDS.  Entries of tile t, line m live in
+// [ptr[t*n_major + m], ptr[t*n_major + m + 1]) with 16-bit in-tile minor
+// indices; every segment is padded with zeros to a multiple of `vec`.
+struct TiledLines {
+    i64 n_major = 0, n_minor = 0;
+    int tile_len = 0, n_tiles = 0;
+    i64 nnz = 0, slots = 0;     // true entries, stored slots (with padding)
+    int vec = 1, lpl = 64;      // vector width per lane, lanes per line
+    int G = 1;                  // nnz-balanced chunks per tile
+    StorageType st = ST_F64;
+    i64* ptr = nullptr;         // device, n_tiles*n_major + 1
+    std::uint16_t* idx = nullptr;
+    void* val = nullptr;
+    int* chunk = nullptr;       // device, n_tiles*(G+1) line boundaries
+};
+
+// ------------------------------------------------------------------ launchers (kernels_*.hip)
+// Sparse build.
+void launch_count_entries(const std::int32_t* idx, const i64* ptr, i64 n_primary, bool transpose, int tile_len,
+                          i64 n_major_out, unsigned long long* counts, hipStream_t s);
+void launch_pad_counts(const unsigned long long* counts, i64* plen, i64 n, int vec, hipStream_t s);
+void exclusive_scan_i64(const i64* in, i64* out, i64 n, hipStream_t s);  // out has n+1 entries
+void launch_scatter_entries(const std::int32_t* idx, const void* val_in, StorageType in_t, const i64* ptr,
+                            i64 n_primary, bool transpose, int tile_len, i64 n_major_out,
+                            unsigned long long* cursor, std::uint16_t* out_idx, void* out_val, StorageType out_t,
+                            hipStream_t s);
+void launch_sort_segments(const i64* ptr, const unsigned long long* counts, i64 n_seg, int tile_len,
+                          const std::uint16_t* in_idx, const void* in_val, std::uint16_t* out_idx, void* out_val,
+                          StorageType st, int* dup_flag, hipStream_t s);
+void launch_chunk_bounds(const i64* ptr, i64 n_major, int n_tiles, int G, int* chunk, hipStream_t s);
+// Sparse products.
+void launch_spmv_tiled(const TiledLines& t, const Prologue& pro, const Epilogue& epi, bool acc64, void* partial,
+                       double* gamma_part, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+void launch_validate_compressed(const std::int32_t* idx, const i64* ptr, i64 n_primary, i64 n_secondary, i64 nnz,
+                                int* flag, hipStream_t s);
+// Realize and column statistics.
+void launch_sparse_realize(const TiledLines& t, bool major_is_col, i64 nr, double* dense, hipStream_t s);
+void launch_sparse_colstats(const TiledLines& t, i64 nr, double* center_out, double* scale_out, hipStream_t s);
+
+// Dense products.
+struct DenseDesc {
+    i64 nr = 0, nc = 0;
+    bool col_major = true;
+    StorageType st = ST_F64;
+    const void* a = nullptr;
+};
+void launch_dense_matvec(const DenseDesc& d, bool adjoint, const Prologue& pro, const Epilogue& epi,
+                         void* xs_scratch, double* gamma_scratch, void* partial, i64 partial_cap, hipStream_t s,
+                         hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+void launch_dense_realize(const DenseDesc& d, double* dense, hipStream_t s);
+void launch_dense_colstats(const DenseDesc& d, double* center_out, double* scale_out, hipStream_t s);
+i64 dense_partial_elems(const DenseDesc& d);
+
+// Lanczos vector kernels (FP64).
+int cgs_blocks(i64 L);
+void launch_cgs_dots(const double* M, i64 ld, int n, const double* f, i64 L, double* partials, hipStream_t s);
+void launch_cgs_reduce(const double* partials, int G, int n, double* t, hipStream_t s);
+void launch_cgs_update(const double* M, i64 ld, int n, const double* t, double* f, i64 L, double* norm_part,
+                       hipStream_t s);
+void launch_sumsq(const double* f, i64 L, double* norm_part, hipStream_t s);
+void launch_normalize(const double* f, i64 L, const double* norm_part, int G, double* norm_out, double* dst,
+                      hipStream_t s);
+void launch_rotate(const double* M, i64 ld, int work, const double* Yt, int ldy, int k, double* out, i64 ld_out,
+                   i64 L, hipStream_t s);
+void launch_scale_copy(const double* src, i64 L, double denom, double* dst, hipStream_t s);
+void launch_vec_combine(double* out, const double* a, const double* b, i64 n, int op, hipStream_t s);
+void launch_axpy_dev(double* y, const double* src, const double* coef, i64 n, hipStream_t s);
+void launch_scale_columns(double* M, i64 ld, i64 L, int k, const double* d, hipStream_t s);
+
+// Synthetic data (counter-based, identical to the oracle's generator).
+void launch_gen_nb_count(i64 n_minor, i64 n_major, std::uint32_t thr, std::uint64_t seed, i64* counts,
+                         hipStream_t s);
+void launch_gen_nb_fill(i64 n_minor, i64 n_major, std::uint32_t thr, std::uint64_t seed, const i64* ptr,
+                        const double* table, bool f32, void* values, std::int32_t* indices, hipStream_t s);
+
+}  // namespace irlba_hip

@@ -1,0 +1,1409 @@
+// Host side of the MI355X IRLBA engine: device matrices, the decorator
+// algebra, the device-resident Lanczos bidiagonalisation, the restart loop
+// with an FP64 host SVD of the small B, pca(), and the C ABI of
+// include/irlba_hip.h.
+//
+// Algorithm: include/irlba/compute.hpp:141-308 (restart loop),
+// include/irlba/lanczos.hpp:54-143 (sweep), include/irlba/pca.hpp:55-139.
+// Every Lanczos vector (V, W, F) lives in HBM in FP64.  A sweep is queued on
+// the context's stream without host round trips: the norms S and R_F are
+// produced and consumed on the device, and the invariant-subspace tests
+// (lanczos.hpp:99-109,123-132) run on the host after the sweep.  If one
+// fires, the affected half-step is recomputed, refilled with host normals,
+// and the rest of the sweep is queued again, which reproduces the
+// reference's step-by-step control flow exactly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <random>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/irlba_hip.h"
+#include "common.hpp"
+
+using namespace irlba_hip;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class Fn>
+int guarded(Fn&& fn) {
+    try {
+        fn();
+        return IRLBA_HIP_OK;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return IRLBA_HIP_ERROR;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return IRLBA_HIP_ERROR;
+    }
+}
+
+void require(bool ok, const char* msg) {
+    if (!ok) throw std::runtime_error(msg);
+}
+
+// ------------------------------------------------------------------ device memory
+template <typename T>
+std::shared_ptr<T> dalloc(i64 n) {
+    T* p = nullptr;
+    const size_t bytes = sizeof(T) * static_cast<size_t>(std::max<i64>(n, 1));
+    IRLBA_HIP_CHECK(hipMalloc(&p, bytes));
+    return std::shared_ptr<T>(p, [](T* q) { (void)hipFree(q); });
+}
+
+std::shared_ptr<void> dalloc_bytes(size_t bytes) {
+    void* p = nullptr;
+    IRLBA_HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 1)));
+    return std::shared_ptr<void>(p, [](void* q) { (void)hipFree(q); });
+}
+
+size_t storage_bytes(StorageType st) { return st == ST_F64 ? 8 : (st == ST_F32 ? 4 : 2); }
+
+StorageType to_storage(int t) {
+    if (t == IRLBA_HIP_F64) return ST_F64;
+    if (t == IRLBA_HIP_F32) return ST_F32;
+    if (t == IRLBA_HIP_BF16) return ST_BF16;
+    throw std::runtime_error("unknown value type");
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ context
+struct irlba_hip_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 256;
+    size_t lds_per_block = 65536;
+    bool profiling = true;
+    irlba_hip_stats stats{};
+    struct Pending {
+        hipEvent_t a, b;
+        bool adjoint;
+    };
+    std::vector<hipEvent_t> pool;
+    std::vector<Pending> pending;
+
+    hipEvent_t get_event() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        IRLBA_HIP_CHECK(hipEventCreate(&e));
+        return e;
+    }
+    // Fold the timings of completed products into the stats (call after a sync).
+    void resolve_events() {
+        for (auto& p : pending) {
+            float ms = 0;
+            // A product with an empty dimension launches nothing and records no events.
+            if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                if (p.adjoint) stats.adjoint_kernel_ms += ms; else stats.forward_kernel_ms += ms;
+            } else {
+                (void)hipGetLastError();  // clear it so the next launch check does not see it
+            }
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void sync() {
+        IRLBA_HIP_CHECK(hipStreamSynchronize(stream));
+        resolve_events();
+    }
+};
+
+namespace {
+
+struct DeviceGuard {
+    explicit DeviceGuard(int dev) { IRLBA_HIP_CHECK(hipSetDevice(dev)); }
+};
+
+// ------------------------------------------------------------------ storage
+struct Base {
+    irlba_hip_context* ctx = nullptr;
+    i64 nr = 0, nc = 0;
+    virtual ~Base() {}
+    virtual void product(bool adjoint, const Prologue& pro, const Epilogue& epi, hipEvent_t e0, hipEvent_t e1) = 0;
+    virtual void realize(double* dense_dev) = 0;      // rows x cols column-major
+    virtual void colstats(double* c, double* s) = 0;  // device outputs, nullable
+    virtual double product_bytes(bool adjoint) const = 0;
+};
+
+struct SparseBase : Base {
+    TiledLines by_row, by_col;  // major = rows (A.v), major = cols (A'.u)
+    std::vector<std::shared_ptr<void> > keep;
+    bool acc64 = true;
+    bool column_major = true;
+    i64 nnz = 0;
+    std::shared_ptr<void> partial_row, partial_col;
+    std::shared_ptr<double> gamma_row, gamma_col;
+
+    void product(bool adjoint, const Prologue& pro, const Epilogue& epi, hipEvent_t e0, hipEvent_t e1) override {
+        const TiledLines& t = adjoint ? by_col : by_row;
+        launch_spmv_tiled(t, pro, epi, acc64, adjoint ? partial_col.get() : partial_row.get(),
+                          adjoint ? gamma_col.get() : gamma_row.get(), ctx->stream, e0, e1);
+    }
+    void realize(double* dense) override {
+        IRLBA_HIP_CHECK(hipMemsetAsync(dense, 0, sizeof(double) * nr * nc, ctx->stream));
+        launch_sparse_realize(by_col, true, nr, dense, ctx->stream);
+    }
+    void colstats(double* c, double* s) override { launch_sparse_colstats(by_col, nr, c, s, ctx->stream); }
+    double product_bytes(bool adjoint) const override {
+        // nnz (value + int32 index) + pointers of the input orientation + input and output vectors (FP64).
+        const double sval = static_cast<double>(storage_bytes(by_row.st));
+        const double nptr = static_cast<double>(column_major ? nc : nr) + 1;
+        const double nin = static_cast<double>(adjoint ? nr : nc), nout = static_cast<double>(adjoint ? nc : nr);
+        return static_cast<double>(nnz) * (sval + 4.0) + nptr * 8.0 + (nin + nout) * 8.0;
+    }
+};
+
+struct DenseBase : Base {
+    DenseDesc d;
+    std::shared_ptr<void> data, xs, partial;
+    std::shared_ptr<double> gamma;
+    i64 partial_cap = 0;
+
+    void product(bool adjoint, const Prologue& pro, const Epilogue& epi, hipEvent_t e0, hipEvent_t e1) override {
+        launch_dense_matvec(d, adjoint, pro, epi, xs.get(), gamma.get(), partial.get(), partial_cap, ctx->stream, e0,
+                            e1);
+    }
+    void realize(double* dense) override { launch_dense_realize(d, dense, ctx->stream); }
+    void colstats(double* c, double* s) override { launch_dense_colstats(d, c, s, ctx->stream); }
+    double product_bytes(bool adjoint) const override {
+        const double nin = static_cast<double>(adjoint ? nr : nc), nout = static_cast<double>(adjoint ? nc : nr);
+        return static_cast<double>(nr) * nc * storage_bytes(d.st) + (nin + nout) * 8.0;
+    }
+};
+
+// The decorators of a handle in closed form:  M = D_r A D_c - sum_k a_k b_k'.
+struct Affine {
+    std::shared_ptr<double> dr, dc;  // nullable
+    int n_terms = 0;
+    std::shared_ptr<double> a[kMaxTerms], b[kMaxTerms];  // a nullable (= ones), b never null
+};
+
+}  // namespace
+
+struct irlba_hip_matrix {
+    irlba_hip_context* ctx = nullptr;
+    std::shared_ptr<Base> base;
+    Affine aff;
+    bool decorated = false;
+};
+
+namespace {
+
+std::shared_ptr<double> upload_vec(irlba_hip_context* ctx, const double* src, i64 n, bool device_ptrs) {
+    auto d = dalloc<double>(n);
+    IRLBA_HIP_CHECK(hipMemcpyAsync(d.get(), src, sizeof(double) * n,
+                                   device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+    IRLBA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return d;
+}
+
+// Apply one product of a handle: y = M x (or M' x), then y -= (*coef) src.
+void apply_handle(const irlba_hip_matrix& m, bool adjoint, const double* x, double* y, const double* axpy_src,
+                  const double* axpy_coef) {
+    Prologue pro;
+    Epilogue epi;
+    const Affine& a = m.aff;
+    pro.x = x;
+    pro.n_terms = a.n_terms;
+    epi.out = y;
+    epi.n_terms = a.n_terms;
+    epi.axpy_src = axpy_src;
+    epi.axpy_coef = axpy_coef;
+    if (!adjoint) {
+        pro.xscale = a.dc.get();
+        epi.oscale = a.dr.get();
+        for (int k = 0; k < a.n_terms; ++k) {
+            pro.tdot[k] = a.b[k].get();
+            epi.tvec[k] = a.a[k].get();
+        }
+    } else {
+        pro.xscale = a.dr.get();
+        epi.oscale = a.dc.get();
+        for (int k = 0; k < a.n_terms; ++k) {
+            pro.tdot[k] = a.a[k].get();
+            epi.tvec[k] = a.b[k].get();
+        }
+    }
+    irlba_hip_context* ctx = m.ctx;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->profiling) {
+        e0 = ctx->get_event();
+        e1 = ctx->get_event();
+        ctx->pending.push_back({e0, e1, adjoint});
+    }
+    if (adjoint) ++ctx->stats.adjoint_products; else ++ctx->stats.forward_products;
+    m.base->product(adjoint, pro, epi, e0, e1);
+}
+
+// ------------------------------------------------------------------ sparse construction
+TiledLines build_lines(irlba_hip_context* ctx, const std::int32_t* d_idx, const void* d_val, StorageType in_t,
+                       const i64* d_ptr, i64 n_primary, bool transpose, i64 n_major_out, i64 n_minor, int tile_len,
+                       StorageType st, i64 nnz, std::vector<std::shared_ptr<void> >& keep) {
+    hipStream_t s = ctx->stream;
+    TiledLines t;
+    t.n_major = n_major_out;
+    t.n_minor = n_minor;
+    t.tile_len = static_cast<int>(std::max<i64>(1, std::min<i64>(tile_len, std::max<i64>(n_minor, 1))));
+    t.n_tiles = static_cast<int>(std::max<i64>(1, (n_minor + t.tile_len - 1) / t.tile_len));
+    t.nnz = nnz;
+    t.st = st;
+    const i64 nseg = static_cast<i64>(t.n_tiles) * n_major_out;
+    require(n_major_out < (static_cast<i64>(1) << 31), "matrix dimension too large for the device layout");
+
+    // Per-lane vector width and lanes per line from the mean segment length.
+    const double avg = nseg ? static_cast<double>(nnz) / nseg : 0.0;
+    t.vec = avg >= 12 ? 4 : 1;
+    const double per = avg / t.vec;
+    int lpl = 4;
+    while (lpl < 64 && lpl * 2 <= per / 2) lpl *= 2;
+    t.lpl = lpl;
+    if (const char* e = std::getenv("IRLBA_HIP_LPL")) t.lpl = std::atoi(e);
+
+    auto counts = dalloc<unsigned long long>(nseg);
+    IRLBA_HIP_CHECK(hipMemsetAsync(counts.get(), 0, sizeof(unsigned long long) * nseg, s));
+    launch_count_entries(d_idx, d_ptr, n_primary, transpose, t.tile_len, n_major_out, counts.get(), s);
+    auto plen = dalloc<i64>(nseg);
+    launch_pad_counts(counts.get(), plen.get(), nseg, t.vec, s);
+    auto ptr = dalloc<i64>(nseg + 1);
+    exclusive_scan_i64(plen.get(), ptr.get(), nseg, s);
+    plen.reset();
+    IRLBA_HIP_CHECK(hipMemcpy(&t.slots, ptr.get() + nseg, sizeof(i64), hipMemcpyDeviceToHost));
+
+    auto cursor = dalloc<unsigned long long>(nseg);
+    IRLBA_HIP_CHECK(hipMemcpyAsync(cursor.get(), ptr.get(), sizeof(i64) * nseg, hipMemcpyDeviceToDevice, s));
+    const size_t vb = storage_bytes(st);
+    auto sidx = dalloc<std::uint16_t>(t.slots);
+    auto sval = dalloc_bytes(vb * t.slots);
+    IRLBA_HIP_CHECK(hipMemsetAsync(sidx.get(), 0, sizeof(std::uint16_t) * t.slots, s));
+    IRLBA_HIP_CHECK(hipMemsetAsync(sval.get(), 0, vb * t.slots, s));
+    launch_scatter_entries(d_idx, d_val, in_t, d_ptr, n_primary, transpose, t.tile_len, n_major_out, cursor.get(),
+                           sidx.get(), sval.get(), st, s);
+    cursor.reset();
+
+    auto fidx = dalloc<std::uint16_t>(t.slots);
+    auto fval = dalloc_bytes(vb * t.slots);
+    IRLBA_HIP_CHECK(hipMemsetAsync(fidx.get(), 0, sizeof(std::uint16_t) * t.slots, s));
+    IRLBA_HIP_CHECK(hipMemsetAsync(fval.get(), 0, vb * t.slots, s));
+    auto dup = dalloc<int>(1);
+    IRLBA_HIP_CHECK(hipMemsetAsync(dup.get(), 0, sizeof(int), s));
+    launch_sort_segments(ptr.get(), counts.get(), nseg, t.tile_len, sidx.get(), sval.get(), fidx.get(), fval.get(), st,
+                         dup.get(), s);
+    int dup_h = 0;
+    IRLBA_HIP_CHECK(hipMemcpyAsync(&dup_h, dup.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+    require(dup_h == 0, "sparse input holds the same row/column pair more than once");
+    sidx.reset();
+    sval.reset();
+    counts.reset();
+
+    // nnz-balanced chunks: enough workgroups to fill every CU once.
+    int target = ctx->cu_count;
+    if (const char* e = std::getenv("IRLBA_HIP_SPMV_BLOCKS")) target = std::max(1, std::atoi(e));
+    t.G = static_cast<int>(std::max<i64>(1, std::min<i64>(std::max<i64>(n_major_out, 1),
+                                                          (target + t.n_tiles - 1) / t.n_tiles)));
+    auto chunk = dalloc<int>(static_cast<i64>(t.n_tiles) * (t.G + 1));
+    launch_chunk_bounds(ptr.get(), n_major_out, t.n_tiles, t.G, chunk.get(), s);
+    IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+
+    t.ptr = ptr.get();
+    t.idx = fidx.get();
+    t.val = fval.get();
+    t.chunk = chunk.get();
+    keep.push_back(ptr);
+    keep.push_back(fidx);
+    keep.push_back(fval);
+    keep.push_back(chunk);
+    return t;
+}
+
+int tile_elems(irlba_hip_context* ctx, bool acc64) {
+    size_t budget = std::min<size_t>(128 * 1024, ctx->lds_per_block > 8192 ? ctx->lds_per_block - 4096 : 4096);
+    if (const char* e = std::getenv("IRLBA_HIP_TILE_BYTES")) budget = std::min<size_t>(budget, std::atol(e));
+    return static_cast<int>(budget / (acc64 ? 8 : 4));
+}
+
+std::shared_ptr<SparseBase> make_sparse(irlba_hip_context* ctx, i64 nrow, i64 ncol, StorageType in_t,
+                                        const void* values, const std::int32_t* indices, const i64* pointers,
+                                        bool column_major, StorageType st, bool acc64, bool device_ptrs) {
+    require(nrow >= 0 && ncol >= 0, "matrix dimensions must be non-negative");
+    require(st == ST_F64 || st == ST_F32, "sparse storage must be F64 or F32");
+    require(!(st == ST_F64 && !acc64), "FP64 storage accumulates in FP64");
+    require(in_t == ST_F64 || in_t == ST_F32, "sparse input values must be F64 or F32");
+    hipStream_t s = ctx->stream;
+    const i64 primary = column_major ? ncol : nrow, secondary = column_major ? nrow : ncol;
+    i64 nnz = 0;
+    if (device_ptrs) IRLBA_HIP_CHECK(hipMemcpy(&nnz, pointers + primary, sizeof(i64), hipMemcpyDeviceToHost));
+    else nnz = pointers[primary];
+    require(nnz >= 0, "sparse pointers must end at a non-negative count");
+
+    std::shared_ptr<void> u_idx, u_val, u_ptr;
+    const std::int32_t* d_idx = indices;
+    const void* d_val = values;
+    const i64* d_ptr = pointers;
+    if (!device_ptrs) {
+        u_idx = dalloc_bytes(sizeof(std::int32_t) * nnz);
+        u_val = dalloc_bytes(storage_bytes(in_t) * nnz);
+        u_ptr = dalloc_bytes(sizeof(i64) * (primary + 1));
+        IRLBA_HIP_CHECK(hipMemcpy(u_idx.get(), indices, sizeof(std::int32_t) * nnz, hipMemcpyHostToDevice));
+        IRLBA_HIP_CHECK(hipMemcpy(u_val.get(), values, storage_bytes(in_t) * nnz, hipMemcpyHostToDevice));
+        IRLBA_HIP_CHECK(hipMemcpy(u_ptr.get(), pointers, sizeof(i64) * (primary + 1), hipMemcpyHostToDevice));
+        d_idx = static_cast<const std::int32_t*>(u_idx.get());
+        d_val = u_val.get();
+        d_ptr = static_cast<const i64*>(u_ptr.get());
+    }
+    {
+        auto flag = dalloc<int>(1);
+        IRLBA_HIP_CHECK(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
+        launch_validate_compressed(d_idx, d_ptr, primary, secondary, nnz, flag.get(), s);
+        int f = 0;
+        IRLBA_HIP_CHECK(hipMemcpyAsync(&f, flag.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+        require(!(f & 1), "sparse pointers must start at 0, be non-decreasing and end at the number of non-zeros");
+        require(!(f & 2), "sparse indices must lie within the secondary dimension");
+    }
+
+    auto sp = std::make_shared<SparseBase>();
+    sp->ctx = ctx;
+    sp->nr = nrow;
+    sp->nc = ncol;
+    sp->nnz = nnz;
+    sp->acc64 = acc64;
+    sp->column_major = column_major;
+    const int tl = tile_elems(ctx, acc64);
+    // by_col: lines are columns, the input (minor) dimension is rows.
+    sp->by_col = build_lines(ctx, d_idx, d_val, in_t, d_ptr, primary, !column_major, ncol, nrow, tl, st, nnz, sp->keep);
+    sp->by_row = build_lines(ctx, d_idx, d_val, in_t, d_ptr, primary, column_major, nrow, ncol, tl, st, nnz, sp->keep);
+    const size_t ab = acc64 ? 8 : 4;
+    sp->partial_row = dalloc_bytes(ab * std::max<i64>(1, sp->by_row.n_tiles > 1 ? sp->by_row.n_tiles * nrow : 1));
+    sp->partial_col = dalloc_bytes(ab * std::max<i64>(1, sp->by_col.n_tiles > 1 ? sp->by_col.n_tiles * ncol : 1));
+    sp->gamma_row = dalloc<double>(2 * sp->by_row.n_tiles);
+    sp->gamma_col = dalloc<double>(2 * sp->by_col.n_tiles);
+    return sp;
+}
+
+// ------------------------------------------------------------------ host FP64 SVD
+// One-sided (Hestenes) Jacobi on the columns of an m x n matrix (m >= n is
+// arranged by transposition), thin factors, values in decreasing order; left
+// vectors of zero values are completed to an orthonormal set.  Stands in for
+// Eigen::JacobiSVD (compute.hpp:27-28).
+struct HostMat {
+    i64 nr = 0, nc = 0;
+    std::vector<double> a;
+    HostMat() = default;
+    HostMat(i64 r, i64 c) : nr(r), nc(c), a(static_cast<size_t>(r * c), 0.0) {}
+    double& operator()(i64 r, i64 c) { return a[r + c * nr]; }
+    double operator()(i64 r, i64 c) const { return a[r + c * nr]; }
+    double* col(i64 c) { return a.data() + c * nr; }
+    const double* col(i64 c) const { return a.data() + c * nr; }
+};
+
+void host_svd(const HostMat& A, HostMat& U, std::vector<double>& S, HostMat& V) {
+    if (A.nr < A.nc) {
+        HostMat T(A.nc, A.nr);
+        for (i64 c = 0; c < A.nc; ++c)
+            for (i64 r = 0; r < A.nr; ++r) T(c, r) = A(r, c);
+        host_svd(T, V, S, U);
+        return;
+    }
+    const i64 m = A.nr, n = A.nc;
+    HostMat G = A, Q(n, n);
+    for (i64 i = 0; i < n; ++i) Q(i, i) = 1;
+    std::vector<double> sq(n);
+    auto colsq = [&](i64 j) {
+        const double* g = G.col(j);
+        double s = 0;
+        for (i64 r = 0; r < m; ++r) s += g[r] * g[r];
+        return s;
+    };
+    for (i64 j = 0; j < n; ++j) sq[j] = colsq(j);
+    const double eps = std::numeric_limits<double>::epsilon();
+    for (int sweep = 0; sweep < 80; ++sweep) {
+        double worst = 0;
+        for (i64 p = 0; p < n - 1; ++p) {
+            for (i64 q = p + 1; q < n; ++q) {
+                double* gp = G.col(p);
+                double* gq = G.col(q);
+                double g = 0;
+                for (i64 r = 0; r < m; ++r) g += gp[r] * gq[r];
+                const double scale = std::sqrt(sq[p] * sq[q]);
+                if (g == 0 || std::abs(g) <= eps * scale) continue;
+                worst = std::max(worst, std::abs(g) / scale);
+                const double zeta = (sq[q] - sq[p]) / (2 * g);
+                const double t = std::copysign(1.0, zeta) / (std::abs(zeta) + std::sqrt(1 + zeta * zeta));
+                const double c = 1 / std::sqrt(1 + t * t), s = c * t;
+                for (i64 r = 0; r < m; ++r) {
+                    const double x = gp[r], y = gq[r];
+                    gp[r] = c * x - s * y;
+                    gq[r] = s * x + c * y;
+                }
+                double* qp = Q.col(p);
+                double* qq = Q.col(q);
+                for (i64 r = 0; r < n; ++r) {
+                    const double x = qp[r], y = qq[r];
+                    qp[r] = c * x - s * y;
+                    qq[r] = s * x + c * y;
+                }
+                sq[p] = colsq(p);
+                sq[q] = colsq(q);
+            }
+        }
+        if (worst <= eps) break;
+    }
+    std::vector<i64> order(n);
+    for (i64 j = 0; j < n; ++j) order[j] = j;
+    std::vector<double> nrm(n);
+    for (i64 j = 0; j < n; ++j) nrm[j] = std::sqrt(colsq(j));
+    std::stable_sort(order.begin(), order.end(), [&](i64 x, i64 y) { return nrm[x] > nrm[y]; });
+    U = HostMat(m, n);
+    V = HostMat(n, n);
+    S.assign(n, 0);
+    std::vector<char> have(n, 0);
+    for (i64 j = 0; j < n; ++j) {
+        const i64 o = order[j];
+        S[j] = nrm[o];
+        std::copy(Q.col(o), Q.col(o) + n, V.col(j));
+        if (nrm[o] > 0) {
+            for (i64 r = 0; r < m; ++r) U(r, j) = G(r, o) / nrm[o];
+            have[j] = 1;
+        }
+    }
+    i64 probe = 0;
+    for (i64 j = 0; j < n; ++j) {
+        if (have[j]) continue;
+        while (probe < m) {
+            std::vector<double> e(m, 0.0);
+            e[probe++] = 1;
+            for (int pass = 0; pass < 2; ++pass)
+                for (i64 k = 0; k < n; ++k) {
+                    if (!have[k]) continue;
+                    double d = 0;
+                    for (i64 r = 0; r < m; ++r) d += U(r, k) * e[r];
+                    for (i64 r = 0; r < m; ++r) e[r] -= d * U(r, k);
+                }
+            double nn = 0;
+            for (double x : e) nn += x * x;
+            nn = std::sqrt(nn);
+            if (nn > 1e-8) {
+                for (i64 r = 0; r < m; ++r) U(r, j) = e[r] / nn;
+                have[j] = 1;
+                break;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ random normals
+// Box-Muller pairs from two uniforms in (0, 1] of std::mt19937_64, the method
+// of aarand::standard_normal used by fill_with_random_normals (utils.hpp:16-37).
+double uniform01(std::mt19937_64& eng) {
+    const double range = static_cast<double>(std::mt19937_64::max() - std::mt19937_64::min()) + 1.0;
+    double v;
+    do {
+        v = static_cast<double>(eng() - std::mt19937_64::min()) / range;
+    } while (v == 0);
+    return v;
+}
+
+void random_normals(double* v, i64 n, std::mt19937_64& eng) {
+    const double two_pi = 6.283185307179586476925286766559;
+    const i64 even = n - (n % 2);
+    for (i64 i = 0; i < even; i += 2) {
+        const double r = std::sqrt(-2.0 * std::log(uniform01(eng)));
+        const double th = two_pi * uniform01(eng);
+        v[i] = r * std::sin(th);
+        v[i + 1] = r * std::cos(th);
+    }
+    if (n % 2) {
+        const double r = std::sqrt(-2.0 * std::log(uniform01(eng)));
+        const double th = two_pi * uniform01(eng);
+        v[even] = r * std::sin(th);
+    }
+}
+
+// ------------------------------------------------------------------ options and helpers
+struct Opts {
+    double invariant_subspace_tolerance = -1;
+    double convergence_tolerance = 1e-5;
+    double singular_value_ratio_tolerance = -1;
+    int extra_work = 7;
+    int max_iterations = 1000;
+    bool exact_for_small_matrix = true;
+    bool exact_for_large_number = true;
+    bool cap_number = false;
+    std::uint64_t seed = std::mt19937_64::default_seed;
+    const double* initial = nullptr;
+    i64 initial_len = 0;
+};
+
+Opts to_opts(const irlba_hip_options* p) {
+    Opts o;
+    if (!p) return o;
+    o.invariant_subspace_tolerance = p->invariant_subspace_tolerance;
+    o.convergence_tolerance = p->convergence_tolerance;
+    o.singular_value_ratio_tolerance = p->singular_value_ratio_tolerance;
+    o.extra_work = p->extra_work;
+    o.max_iterations = p->max_iterations;
+    o.exact_for_small_matrix = p->exact_for_small_matrix != 0;
+    o.exact_for_large_number = p->exact_for_large_number != 0;
+    o.cap_number = p->cap_number != 0;
+    o.seed = p->seed;
+    o.initial = p->initial;
+    o.initial_len = p->initial_length;
+    return o;
+}
+
+// compute.hpp:49-56
+bool requested_ge_half(i64 requested, i64 smaller) {
+    const i64 half = smaller / 2;
+    if (requested == half) return smaller % 2 == 0;
+    return requested > half;
+}
+// compute.hpp:59-67
+i64 choose_work(i64 requested, int extra_work, i64 smaller) {
+    if (requested >= smaller) return smaller;
+    return requested + std::min<i64>(smaller - requested, extra_work);
+}
+// compute.hpp:78-111
+i64 update_k(i64 k, i64 requested, i64 n_converged, i64 work) {
+    if (work <= 3) return 1;
+    const i64 limit = work - 3;
+    auto below = [&](i64 v) { return v < requested || (v - requested) < n_converged; };
+    if (below(k)) {
+        if (below(limit)) return limit;
+        return std::max<i64>(requested + n_converged, 1);
+    }
+    return std::max<i64>(std::min(k, limit), 1);
+}
+
+// ------------------------------------------------------------------ operators seen by the engine
+struct Op {
+    irlba_hip_context* ctx;
+    i64 nr, nc;
+    virtual ~Op() {}
+    virtual void apply(bool adjoint, const double* x, double* y, const double* src, const double* coef) = 0;
+    virtual void realize_host(HostMat& out) = 0;
+};
+
+struct HandleOp : Op {
+    const irlba_hip_matrix* m;
+    void apply(bool adjoint, const double* x, double* y, const double* src, const double* coef) override {
+        apply_handle(*m, adjoint, x, y, src, coef);
+    }
+    void realize_host(HostMat& out) override;
+};
+
+struct CallbackOp : Op {
+    irlba_hip_matvec_fn fn = nullptr;
+    void* user = nullptr;
+    std::vector<double> hin, hout;
+    void apply(bool adjoint, const double* x, double* y, const double* src, const double* coef) override {
+        const i64 nin = adjoint ? nr : nc, nout = adjoint ? nc : nr;
+        hin.resize(nin);
+        hout.resize(nout);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(hin.data(), x, sizeof(double) * nin, hipMemcpyDeviceToHost, ctx->stream));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (fn(user, adjoint ? 1 : 0, hin.data(), hout.data()) != 0)
+            throw std::runtime_error("matrix product callback failed");
+        if (adjoint) ++ctx->stats.adjoint_products; else ++ctx->stats.forward_products;
+        IRLBA_HIP_CHECK(hipMemcpyAsync(y, hout.data(), sizeof(double) * nout, hipMemcpyHostToDevice, ctx->stream));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (src) launch_axpy_dev(y, src, coef, nout, ctx->stream);
+    }
+    void realize_host(HostMat& out) override {
+        out = HostMat(nr, nc);
+        std::vector<double> e(nc, 0.0);
+        for (i64 c = 0; c < nc; ++c) {
+            e[c] = 1;
+            if (fn(user, 0, e.data(), out.col(c)) != 0) throw std::runtime_error("matrix product callback failed");
+            e[c] = 0;
+        }
+    }
+};
+
+void HandleOp::realize_host(HostMat& out) {
+    auto dense = dalloc<double>(nr * nc);
+    IRLBA_HIP_CHECK(hipMemsetAsync(dense.get(), 0, sizeof(double) * nr * nc, ctx->stream));
+    m->base->realize(dense.get());
+    // Decorators on the realized matrix (centered.hpp:118-122, scaled.hpp:158-177) on the host.
+    out = HostMat(nr, nc);
+    IRLBA_HIP_CHECK(hipMemcpyAsync(out.a.data(), dense.get(), sizeof(double) * nr * nc, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    IRLBA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const Affine& a = m->aff;
+    auto fetch = [&](const std::shared_ptr<double>& v, i64 n) {
+        std::vector<double> h;
+        if (v) {
+            h.resize(n);
+            IRLBA_HIP_CHECK(hipMemcpy(h.data(), v.get(), sizeof(double) * n, hipMemcpyDeviceToHost));
+        }
+        return h;
+    };
+    const auto dr = fetch(a.dr, nr), dc = fetch(a.dc, nc);
+    for (i64 c = 0; c < nc; ++c)
+        for (i64 r = 0; r < nr; ++r) {
+            double v = out(r, c);
+            if (!dr.empty()) v *= dr[r];
+            if (!dc.empty()) v *= dc[c];
+            out(r, c) = v;
+        }
+    for (int k = 0; k < a.n_terms; ++k) {
+        const auto av = fetch(a.a[k], nr), bv = fetch(a.b[k], nc);
+        for (i64 c = 0; c < nc; ++c)
+            for (i64 r = 0; r < nr; ++r) out(r, c) -= (av.empty() ? 1.0 : av[r]) * bv[c];
+    }
+}
+
+// ------------------------------------------------------------------ the engine
+struct Engine {
+    irlba_hip_context* ctx;
+    Op& op;
+    const Opts& o;
+    hipStream_t s;
+    i64 nr, nc, work;
+    double eps;
+    std::shared_ptr<double> V, W, F, Wn, dnorm, partials, tvec, norm_part, Yt, rot_tmp;
+    int ldy = 0;
+    std::mt19937_64 eng;
+    std::vector<double> S, R;  // host copies of the sweep norms
+    int refills = 0;
+
+    Engine(irlba_hip_context* c, Op& op_, const Opts& o_, i64 work_)
+        : ctx(c), op(op_), o(o_), s(c->stream), nr(op_.nr), nc(op_.nc), work(work_), eng(o_.seed) {
+        const double raw = o.invariant_subspace_tolerance;
+        eps = raw < 0 ? std::pow(std::numeric_limits<double>::epsilon(), 0.8) : raw;
+        V = dalloc<double>(nc * work);
+        W = dalloc<double>(nr * work);
+        F = dalloc<double>(nc);
+        Wn = dalloc<double>(nr);
+        dnorm = dalloc<double>(2 * work + 4);
+        const int G = std::max(cgs_blocks(nr), cgs_blocks(nc));
+        partials = dalloc<double>(static_cast<i64>(G) * work);
+        tvec = dalloc<double>(work);
+        norm_part = dalloc<double>(G);
+        ldy = static_cast<int>((work + 15) / 16 * 16);
+        Yt = dalloc<double>(work * ldy);
+        IRLBA_HIP_CHECK(hipMemsetAsync(W.get(), 0, sizeof(double) * nr * work, s));
+        IRLBA_HIP_CHECK(hipMemsetAsync(V.get(), 0, sizeof(double) * nc * work, s));
+        IRLBA_HIP_CHECK(hipMemsetAsync(dnorm.get(), 0, sizeof(double) * (2 * work + 4), s));
+        S.assign(work, 0.0);
+        R.assign(work, 0.0);
+    }
+
+    double* Vc(i64 j) { return V.get() + j * nc; }
+    double* Wc(i64 j) { return W.get() + j * nr; }
+    double* dS(i64 j) { return dnorm.get() + j; }
+    double* dR(i64 j) { return dnorm.get() + work + 1 + j; }
+    double* dTmp() { return dnorm.get() + 2 * work + 2; }
+
+    // f -= M[:, :n] M[:, :n]' f (lanczos.hpp:17-21); leaves the sums of squares of f in norm_part.
+    void cgs(double* M, i64 L, i64 n, double* f) {
+        if (n == 0) {
+            launch_sumsq(f, L, norm_part.get(), s);
+            return;
+        }
+        launch_cgs_dots(M, L, static_cast<int>(n), f, L, partials.get(), s);
+        launch_cgs_reduce(partials.get(), cgs_blocks(L), static_cast<int>(n), tvec.get(), s);
+        launch_cgs_update(M, L, static_cast<int>(n), tvec.get(), f, L, norm_part.get(), s);
+    }
+    void norm_to(double* f, i64 L, double* norm_dev, double* dst) {
+        launch_normalize(f, L, norm_part.get(), cgs_blocks(L), norm_dev, dst, s);
+    }
+
+    // lanczos.hpp:73-86
+    void first(i64 start) {
+        op.apply(false, Vc(start), Wn.get(), nullptr, nullptr);
+        cgs(W.get(), nr, start, Wn.get());
+        norm_to(Wn.get(), nr, dS(start), Wc(start));
+    }
+    // lanczos.hpp:93-113 (half-step 2j)
+    void phase_a(i64 j) {
+        op.apply(true, Wc(j), F.get(), Vc(j), dS(j));
+        cgs(V.get(), nc, j + 1, F.get());
+        norm_to(F.get(), nc, dR(j), j + 1 < work ? Vc(j + 1) : nullptr);
+    }
+    // lanczos.hpp:115-134 (half-step 2j+1)
+    void phase_b(i64 j) {
+        op.apply(false, Vc(j + 1), Wn.get(), Wc(j), dR(j));
+        cgs(W.get(), nr, j + 1, Wn.get());
+        norm_to(Wn.get(), nr, dS(j + 1), Wc(j + 1));
+    }
+    void launch_from(i64 h) {
+        const i64 hend = 2 * (work - 1);  // the last half-step is phase A of j = work-1
+        for (; h <= hend; ++h) {
+            if (h % 2 == 0) phase_a(h / 2); else phase_b(h / 2);
+        }
+    }
+    void read_norms() {
+        std::vector<double> h(2 * work + 4);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(h.data(), dnorm.get(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+        ctx->sync();
+        for (i64 j = 0; j < work; ++j) {
+            S[j] = h[j];
+            R[j] = h[work + 1 + j];
+        }
+    }
+    void set_dev_scalar(double* d, double v) {
+        IRLBA_HIP_CHECK(hipMemcpyAsync(d, &v, sizeof(double), hipMemcpyHostToDevice, s));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    void upload_normals(double* dst, i64 n) {
+        std::vector<double> h(n);
+        random_normals(h.data(), n, eng);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(dst, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    // run_lanczos_bidiagonalization (lanczos.hpp:54-143) from column `start`; fills B.
+    void sweep(i64 start, HostMat& B) {
+        first(start);
+        launch_from(2 * start);
+        read_norms();
+        if (S[start] < eps) throw std::runtime_error("starting vector near the null space of the input matrix");
+        i64 h = 2 * start;
+        const i64 hcheck_end = 2 * (work - 1);  // half-steps with an invariant-subspace test
+        while (true) {
+            i64 ev = -1;
+            for (; h < hcheck_end; ++h) {
+                const i64 j = h / 2;
+                const double nrm = (h % 2 == 0) ? R[j] : S[j + 1];
+                if (nrm < eps) {
+                    ev = h;
+                    break;
+                }
+            }
+            if (ev < 0) break;
+            const i64 j = ev / 2;
+            ++refills;
+            if (ev % 2 == 0) {
+                // lanczos.hpp:101-106: recompute F, refill with normals, re-orthogonalise, R_F = 0.
+                op.apply(true, Wc(j), F.get(), Vc(j), dS(j));
+                cgs(V.get(), nc, j + 1, F.get());
+                upload_normals(F.get(), nc);
+                cgs(V.get(), nc, j + 1, F.get());
+                norm_to(F.get(), nc, dTmp(), Vc(j + 1));
+                set_dev_scalar(dR(j), 0.0);
+            } else {
+                // lanczos.hpp:124-130: the same for W_next, S = 0.
+                op.apply(false, Vc(j + 1), Wn.get(), Wc(j), dR(j));
+                cgs(W.get(), nr, j + 1, Wn.get());
+                upload_normals(Wn.get(), nr);
+                cgs(W.get(), nr, j + 1, Wn.get());
+                norm_to(Wn.get(), nr, dTmp(), Wc(j + 1));
+                set_dev_scalar(dS(j + 1), 0.0);
+            }
+            launch_from(ev + 1);
+            read_norms();
+            h = ev + 1;
+        }
+        for (i64 j = start; j < work; ++j) {
+            B(j, j) = S[j];
+            if (j + 1 < work) B(j, j + 1) = R[j];
+        }
+    }
+
+    // dst[:, :k] = src[:, :work] Y[:, :k] on the matrix cores.
+    void rotate(double* M, i64 L, const HostMat& Y, i64 k, double* out, i64 ld_out) {
+        std::vector<double> yt(static_cast<size_t>(work * ldy), 0.0);
+        for (i64 r = 0; r < work; ++r)
+            for (i64 c = 0; c < k; ++c) yt[r * ldy + c] = Y(r, c);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(Yt.get(), yt.data(), sizeof(double) * yt.size(), hipMemcpyHostToDevice, s));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));  // yt is a host temporary
+        if (out == M && k > 128) {
+            if (!rot_tmp) rot_tmp = dalloc<double>(std::max(nr, nc) * work);
+            launch_rotate(M, L, static_cast<int>(work), Yt.get(), ldy, static_cast<int>(k), rot_tmp.get(), L, L, s);
+            IRLBA_HIP_CHECK(hipMemcpyAsync(M, rot_tmp.get(), sizeof(double) * L * k, hipMemcpyDeviceToDevice, s));
+        } else {
+            launch_rotate(M, L, static_cast<int>(work), Yt.get(), ldy, static_cast<int>(k), out, ld_out, L, s);
+        }
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+    }
+};
+
+// Exact SVD of the realized matrix (compute.hpp:30-46).
+void exact_svd(Op& op, i64 requested, double* U, double* V, double* D, bool device_ptrs, hipStream_t s) {
+    HostMat A;
+    op.realize_host(A);
+    HostMat u, v;
+    std::vector<double> sv;
+    host_svd(A, u, sv, v);
+    const auto kind = device_ptrs ? hipMemcpyHostToDevice : hipMemcpyHostToHost;
+    IRLBA_HIP_CHECK(hipMemcpyAsync(D, sv.data(), sizeof(double) * requested, kind, s));
+    IRLBA_HIP_CHECK(hipMemcpyAsync(U, u.a.data(), sizeof(double) * u.nr * requested, kind, s));
+    IRLBA_HIP_CHECK(hipMemcpyAsync(V, v.a.data(), sizeof(double) * v.nr * requested, kind, s));
+    IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// compute.hpp:141-308.  Outputs sized for `number`; returns (converged, iterations).
+std::pair<bool, int> run_compute(irlba_hip_context* ctx, Op& op, i64 number, const Opts& o, double* U, double* V,
+                                 double* D, i64* nout, bool device_ptrs) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const i64 nr = op.nr, nc = op.nc;
+    const i64 smaller = std::min(nr, nc);
+    i64 requested = number;
+    if (requested > smaller) {
+        if (o.cap_number) requested = smaller;
+        else throw std::runtime_error("requested number of singular values cannot be greater than the smaller matrix dimension");
+    } else if (requested == smaller && !o.exact_for_large_number) {
+        throw std::runtime_error("requested number of singular values must be less than the smaller matrix dimension for IRLBA iterations");
+    }
+    *nout = requested;
+    auto& st = ctx->stats;
+    st = irlba_hip_stats{};
+    if ((o.exact_for_small_matrix && smaller < 6) || (o.exact_for_large_number && requested_ge_half(requested, smaller))) {
+        exact_svd(op, requested, U, V, D, device_ptrs, ctx->stream);
+        st.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return {true, 0};
+    }
+    const i64 work = choose_work(requested, o.extra_work, smaller);
+    if (work == 0) throw std::runtime_error("number of requested dimensions must be positive");
+
+    hipStream_t s = ctx->stream;
+    Engine eg(ctx, op, o, work);
+    {
+        std::vector<double> v0(nc);
+        if (o.initial) {
+            if (o.initial_len != nc)
+                throw std::runtime_error("initialization vector does not have expected number of rows");
+            std::copy(o.initial, o.initial + nc, v0.begin());
+        } else {
+            random_normals(v0.data(), nc, eg.eng);
+        }
+        IRLBA_HIP_CHECK(hipMemcpyAsync(eg.Vc(0), v0.data(), sizeof(double) * nc, hipMemcpyHostToDevice, s));
+        launch_sumsq(eg.Vc(0), nc, eg.norm_part.get(), s);
+        eg.norm_to(eg.Vc(0), nc, eg.dTmp(), eg.Vc(0));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    bool converged = false;
+    int iter = 0;
+    i64 k = 0;
+    HostMat B(work, work), BU, BV;
+    std::vector<double> BS, res(work), prevS(work, 0.0);
+    const double tol = o.convergence_tolerance;
+    const double svtol = o.singular_value_ratio_tolerance >= 0 ? o.singular_value_ratio_tolerance : tol;
+
+    for (; iter < o.max_iterations; ++iter) {
+        eg.sweep(k, B);
+        const auto ts = std::chrono::steady_clock::now();
+        host_svd(B, BU, BS, BV);
+        st.host_svd_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
+        ++st.restarts;
+        if (B(work - 1, work - 1) == 0) {
+            converged = true;
+            break;
+        }
+        const double R_F = eg.R[work - 1];
+        for (i64 j = 0; j < work; ++j) res[j] = R_F * BU(work - 1, j);
+        i64 n_conv = 0;
+        if (iter > 0) {
+            const double Smax = *std::max_element(BS.begin(), BS.end());
+            const double thr = Smax * tol;
+            for (i64 j = 0; j < work; ++j) {
+                if (std::abs(res[j]) <= thr) {
+                    const double ratio = std::abs(BS[j] - prevS[j]) / BS[j];
+                    if (ratio <= svtol) ++n_conv;
+                }
+            }
+            if (n_conv >= requested) {
+                converged = true;
+                break;
+            }
+        }
+        prevS = BS;
+        k = update_k(k, requested, n_conv, work);
+        eg.rotate(eg.V.get(), nc, BV, k, eg.V.get(), nc);
+        launch_scale_copy(eg.F.get(), nc, R_F, eg.Vc(k), s);
+        eg.rotate(eg.W.get(), nr, BU, k, eg.W.get(), nr);
+        std::fill(B.a.begin(), B.a.end(), 0.0);
+        for (i64 l = 0; l < k; ++l) {
+            B(l, l) = BS[l];
+            B(l, k) = res[l];
+        }
+    }
+
+    // compute.hpp:298-305 (with the last B-SVD, as the reference does).
+    if (BS.empty()) {
+        BS.assign(work, 0.0);
+        BU = HostMat(work, work);
+        BV = HostMat(work, work);
+    }
+    std::shared_ptr<double> dU, dV;
+    double* oU = U;
+    double* oV = V;
+    if (!device_ptrs) {
+        dU = dalloc<double>(nr * requested);
+        dV = dalloc<double>(nc * requested);
+        oU = dU.get();
+        oV = dV.get();
+    }
+    if (requested > 0) {
+        eg.rotate(eg.W.get(), nr, BU, requested, oU, nr);
+        eg.rotate(eg.V.get(), nc, BV, requested, oV, nc);
+    }
+    const auto kind = device_ptrs ? hipMemcpyHostToDevice : hipMemcpyHostToHost;
+    IRLBA_HIP_CHECK(hipMemcpyAsync(D, BS.data(), sizeof(double) * requested, kind, s));
+    if (!device_ptrs && requested > 0) {
+        IRLBA_HIP_CHECK(hipMemcpyAsync(U, oU, sizeof(double) * nr * requested, hipMemcpyDeviceToHost, s));
+        IRLBA_HIP_CHECK(hipMemcpyAsync(V, oV, sizeof(double) * nc * requested, hipMemcpyDeviceToHost, s));
+    }
+    ctx->sync();
+    st.refills = eg.refills;
+    st.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return {converged, converged ? iter + 1 : iter};
+}
+
+// ------------------------------------------------------------------ decorator composition
+std::shared_ptr<double> combine(irlba_hip_context* ctx, const double* a, const double* b, i64 n, int op) {
+    auto out = dalloc<double>(n);
+    launch_vec_combine(out.get(), a, b, n, op, ctx->stream);
+    return out;
+}
+
+irlba_hip_matrix* decorate_center(const irlba_hip_matrix* in, std::shared_ptr<double> c) {
+    auto* m = new irlba_hip_matrix(*in);
+    m->decorated = true;
+    Affine& a = m->aff;
+    for (int k = 0; k < a.n_terms; ++k) {
+        if (!a.a[k]) {  // merge with the existing ones-term: b += c
+            a.b[k] = combine(m->ctx, a.b[k].get(), c.get(), m->base->nc, 2);
+            IRLBA_HIP_CHECK(hipStreamSynchronize(m->ctx->stream));
+            return m;
+        }
+    }
+    if (a.n_terms >= kMaxTerms) {
+        delete m;
+        throw std::runtime_error("decorator chain has too many centring terms for the device operator");
+    }
+    a.a[a.n_terms] = nullptr;
+    a.b[a.n_terms] = std::move(c);
+    ++a.n_terms;
+    return m;
+}
+
+irlba_hip_matrix* decorate_scale(const irlba_hip_matrix* in, const std::shared_ptr<double>& sdev, bool column,
+                                 bool divide) {
+    auto* m = new irlba_hip_matrix(*in);
+    m->decorated = true;
+    irlba_hip_context* ctx = m->ctx;
+    const i64 n = column ? m->base->nc : m->base->nr;
+    auto f = divide ? combine(ctx, sdev.get(), nullptr, n, 3) : sdev;  // 1/s or s
+    Affine& a = m->aff;
+    if (column) {
+        a.dc = a.dc ? combine(ctx, a.dc.get(), f.get(), n, 0) : f;
+        for (int k = 0; k < a.n_terms; ++k) a.b[k] = combine(ctx, a.b[k].get(), f.get(), n, 0);
+    } else {
+        a.dr = a.dr ? combine(ctx, a.dr.get(), f.get(), n, 0) : f;
+        for (int k = 0; k < a.n_terms; ++k) a.a[k] = a.a[k] ? combine(ctx, a.a[k].get(), f.get(), n, 0) : f;
+    }
+    IRLBA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return m;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+void irlba_hip_options_default(irlba_hip_options* opt) {
+    if (!opt) return;
+    std::memset(opt, 0, sizeof(*opt));
+    opt->invariant_subspace_tolerance = -1;
+    opt->convergence_tolerance = 1e-5;
+    opt->singular_value_ratio_tolerance = -1;
+    opt->extra_work = 7;
+    opt->max_iterations = 1000;
+    opt->exact_for_small_matrix = 1;
+    opt->exact_for_large_number = 1;
+    opt->cap_number = 0;
+    opt->seed = std::mt19937_64::default_seed;
+    opt->initial = nullptr;
+    opt->initial_length = 0;
+}
+
+const char* irlba_hip_last_error(void) { return g_last_error.c_str(); }
+
+int irlba_hip_context_create(int device, irlba_hip_context** out) {
+    return guarded([&] {
+        require(out != nullptr, "null output pointer");
+        int n = 0;
+        IRLBA_HIP_CHECK(hipGetDeviceCount(&n));
+        require(device >= 0 && device < n, "no such HIP device");
+        DeviceGuard g(device);
+        auto ctx = std::make_unique<irlba_hip_context>();
+        ctx->device = device;
+        IRLBA_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        IRLBA_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+        ctx->cu_count = prop.multiProcessorCount;
+        ctx->lds_per_block = prop.sharedMemPerBlock;
+        *out = ctx.release();
+    });
+}
+
+int irlba_hip_context_destroy(irlba_hip_context* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        DeviceGuard g(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto& p : ctx->pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (auto e : ctx->pool) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+int irlba_hip_context_synchronize(irlba_hip_context* ctx) {
+    return guarded([&] {
+        DeviceGuard g(ctx->device);
+        ctx->sync();
+    });
+}
+
+int irlba_hip_get_stats(const irlba_hip_context* ctx, irlba_hip_stats* out) {
+    return guarded([&] { *out = ctx->stats; });
+}
+
+int irlba_hip_context_set_profiling(irlba_hip_context* ctx, int enabled) {
+    return guarded([&] { ctx->profiling = enabled != 0; });
+}
+
+void* irlba_hip_context_stream(irlba_hip_context* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int irlba_hip_sparse_create(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, int input_type, const void* values,
+                            const int32_t* indices, const int64_t* pointers, int column_major, int storage_type,
+                            int accumulate_f64, int device_pointers, irlba_hip_matrix** out) {
+    return guarded([&] {
+        require(ctx && out, "null argument");
+        DeviceGuard g(ctx->device);
+        auto base = make_sparse(ctx, nrow, ncol, to_storage(input_type), values, indices, pointers, column_major != 0,
+                                to_storage(storage_type), accumulate_f64 != 0, device_pointers != 0);
+        auto* m = new irlba_hip_matrix;
+        m->ctx = ctx;
+        m->base = base;
+        *out = m;
+    });
+}
+
+int irlba_hip_dense_create(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, int input_type, const void* values,
+                           int column_major, int storage_type, int device_pointers, irlba_hip_matrix** out) {
+    return guarded([&] {
+        require(ctx && out, "null argument");
+        require(nrow >= 0 && ncol >= 0, "matrix dimensions must be non-negative");
+        DeviceGuard g(ctx->device);
+        const StorageType in_t = to_storage(input_type), st = to_storage(storage_type);
+        auto d = std::make_shared<DenseBase>();
+        d->ctx = ctx;
+        d->nr = nrow;
+        d->nc = ncol;
+        const i64 n = nrow * ncol;
+        d->data = dalloc_bytes(storage_bytes(st) * n);
+        if (in_t == st) {
+            IRLBA_HIP_CHECK(hipMemcpy(d->data.get(), values, storage_bytes(st) * n,
+                                      device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+        } else {
+            // Convert on the device through the sparse scatter's value conversion: stage as given.
+            auto raw = dalloc_bytes(storage_bytes(in_t) * n);
+            IRLBA_HIP_CHECK(hipMemcpy(raw.get(), values, storage_bytes(in_t) * n,
+                                      device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+            std::vector<char> tmp(storage_bytes(in_t) * n);
+            IRLBA_HIP_CHECK(hipMemcpy(tmp.data(), raw.get(), tmp.size(), hipMemcpyDeviceToHost));
+            std::vector<char> conv(storage_bytes(st) * n);
+            for (i64 i = 0; i < n; ++i) {
+                const double v = in_t == ST_F64 ? reinterpret_cast<const double*>(tmp.data())[i]
+                                                : static_cast<double>(reinterpret_cast<const float*>(tmp.data())[i]);
+                if (st == ST_F64) reinterpret_cast<double*>(conv.data())[i] = v;
+                else if (st == ST_F32) reinterpret_cast<float*>(conv.data())[i] = static_cast<float>(v);
+                else {
+                    // round-to-nearest-even FP32 -> BF16
+                    float f = static_cast<float>(v);
+                    std::uint32_t u;
+                    std::memcpy(&u, &f, 4);
+                    std::uint16_t b;
+                    if ((u & 0x7fffffffu) > 0x7f800000u) b = static_cast<std::uint16_t>((u >> 16) | 0x40u);
+                    else b = static_cast<std::uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+                    reinterpret_cast<std::uint16_t*>(conv.data())[i] = b;
+                }
+            }
+            IRLBA_HIP_CHECK(hipMemcpy(d->data.get(), conv.data(), conv.size(), hipMemcpyHostToDevice));
+        }
+        d->d.nr = nrow;
+        d->d.nc = ncol;
+        d->d.col_major = column_major != 0;
+        d->d.st = st;
+        d->d.a = d->data.get();
+        const size_t ab = st == ST_F64 ? 8 : 4;
+        d->xs = dalloc_bytes(ab * std::max<i64>(1, std::max(nrow, ncol)));
+        d->partial_cap = dense_partial_elems(d->d);
+        d->partial = dalloc_bytes(ab * std::max<i64>(1, d->partial_cap));
+        d->gamma = dalloc<double>(2 * 256);
+        auto* m = new irlba_hip_matrix;
+        m->ctx = ctx;
+        m->base = d;
+        *out = m;
+    });
+}
+
+int irlba_hip_centered_create(const irlba_hip_matrix* inner, const double* center, int device_pointers,
+                              irlba_hip_matrix** out) {
+    return guarded([&] {
+        require(inner && out && center, "null argument");
+        DeviceGuard g(inner->ctx->device);
+        *out = decorate_center(inner, upload_vec(inner->ctx, center, inner->base->nc, device_pointers != 0));
+    });
+}
+
+int irlba_hip_scaled_create(const irlba_hip_matrix* inner, const double* scale, int column, int divide,
+                            int device_pointers, irlba_hip_matrix** out) {
+    return guarded([&] {
+        require(inner && out && scale, "null argument");
+        DeviceGuard g(inner->ctx->device);
+        const i64 n = column ? inner->base->nc : inner->base->nr;
+        *out = decorate_scale(inner, upload_vec(inner->ctx, scale, n, device_pointers != 0), column != 0, divide != 0);
+    });
+}
+
+int irlba_hip_matrix_destroy(irlba_hip_matrix* m) {
+    return guarded([&] {
+        if (!m) return;
+        DeviceGuard g(m->ctx->device);
+        IRLBA_HIP_CHECK(hipStreamSynchronize(m->ctx->stream));
+        delete m;
+    });
+}
+
+int irlba_hip_matrix_dims(const irlba_hip_matrix* m, int64_t* rows, int64_t* cols) {
+    return guarded([&] {
+        require(m != nullptr, "null matrix");
+        if (rows) *rows = m->base->nr;
+        if (cols) *cols = m->base->nc;
+    });
+}
+
+int irlba_hip_multiply(const irlba_hip_matrix* m, int adjoint, const double* right, double* output,
+                       int device_pointers) {
+    return guarded([&] {
+        require(m && right && output, "null argument");
+        irlba_hip_context* ctx = m->ctx;
+        DeviceGuard g(ctx->device);
+        const i64 nin = adjoint ? m->base->nr : m->base->nc, nout = adjoint ? m->base->nc : m->base->nr;
+        if (device_pointers) {
+            apply_handle(*m, adjoint != 0, right, output, nullptr, nullptr);
+            ctx->sync();
+            return;
+        }
+        auto din = dalloc<double>(nin);
+        auto dout = dalloc<double>(nout);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(din.get(), right, sizeof(double) * nin, hipMemcpyHostToDevice, ctx->stream));
+        apply_handle(*m, adjoint != 0, din.get(), dout.get(), nullptr, nullptr);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(output, dout.get(), sizeof(double) * nout, hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+    });
+}
+
+int irlba_hip_realize(const irlba_hip_matrix* m, double* buffer, int device_pointers) {
+    return guarded([&] {
+        require(m && buffer, "null argument");
+        DeviceGuard g(m->ctx->device);
+        HandleOp op;
+        op.ctx = m->ctx;
+        op.nr = m->base->nr;
+        op.nc = m->base->nc;
+        op.m = m;
+        HostMat A;
+        op.realize_host(A);
+        IRLBA_HIP_CHECK(hipMemcpy(buffer, A.a.data(), sizeof(double) * A.a.size(),
+                                  device_pointers ? hipMemcpyHostToDevice : hipMemcpyHostToHost));
+    });
+}
+
+int irlba_hip_compute(const irlba_hip_matrix* m, int64_t number, const irlba_hip_options* opt, double* U, double* V,
+                      double* D, int64_t* number_out, int32_t* converged, int32_t* iterations, int device_pointers) {
+    return guarded([&] {
+        require(m != nullptr, "null matrix");
+        DeviceGuard g(m->ctx->device);
+        HandleOp op;
+        op.ctx = m->ctx;
+        op.nr = m->base->nr;
+        op.nc = m->base->nc;
+        op.m = m;
+        const Opts o = to_opts(opt);
+        i64 nout = 0;
+        auto r = run_compute(m->ctx, op, number, o, U, V, D, &nout, device_pointers != 0);
+        m->ctx->stats.forward_bytes = m->base->product_bytes(false);
+        m->ctx->stats.adjoint_bytes = m->base->product_bytes(true);
+        if (number_out) *number_out = nout;
+        if (converged) *converged = r.first;
+        if (iterations) *iterations = r.second;
+    });
+}
+
+int irlba_hip_compute_callback(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, irlba_hip_matvec_fn matvec,
+                               void* user, int64_t number, const irlba_hip_options* opt, double* U, double* V,
+                               double* D, int64_t* number_out, int32_t* converged, int32_t* iterations) {
+    return guarded([&] {
+        require(ctx && matvec, "null argument");
+        DeviceGuard g(ctx->device);
+        CallbackOp op;
+        op.ctx = ctx;
+        op.nr = nrow;
+        op.nc = ncol;
+        op.fn = matvec;
+        op.user = user;
+        const Opts o = to_opts(opt);
+        i64 nout = 0;
+        auto r = run_compute(ctx, op, number, o, U, V, D, &nout, false);
+        if (number_out) *number_out = nout;
+        if (converged) *converged = r.first;
+        if (iterations) *iterations = r.second;
+    });
+}
+
+int irlba_hip_column_stats(const irlba_hip_matrix* m, int center, int scale, double* center_out, double* scale_out,
+                           int device_pointers) {
+    return guarded([&] {
+        require(m != nullptr, "null matrix");
+        require(!m->decorated, "column statistics are computed on an undecorated matrix");
+        irlba_hip_context* ctx = m->ctx;
+        DeviceGuard g(ctx->device);
+        const i64 nr = m->base->nr, nc = m->base->nc;
+        if (center && nr < 1) throw std::runtime_error("cannot center with no observations");
+        if (scale && nr < 2) throw std::runtime_error("cannot scale with fewer than two observations");
+        auto c = dalloc<double>(nc), s = dalloc<double>(nc);
+        m->base->colstats(center ? c.get() : nullptr, scale ? s.get() : nullptr);
+        const auto kind = device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        if (center) IRLBA_HIP_CHECK(hipMemcpyAsync(center_out, c.get(), sizeof(double) * nc, kind, ctx->stream));
+        if (scale) IRLBA_HIP_CHECK(hipMemcpyAsync(scale_out, s.get(), sizeof(double) * nc, kind, ctx->stream));
+        ctx->sync();
+    });
+}
+
+int irlba_hip_pca(const irlba_hip_matrix* m, int center, int scale, int64_t number, const irlba_hip_options* opt,
+                  double* scores, double* rotation, double* variances, int64_t* number_out, int32_t* converged,
+                  int32_t* iterations, int device_pointers) {
+    return guarded([&] {
+        require(m != nullptr, "null matrix");
+        require(!m->decorated, "pca() takes an undecorated matrix");
+        irlba_hip_context* ctx = m->ctx;
+        DeviceGuard g(ctx->device);
+        const i64 nr = m->base->nr, nc = m->base->nc;
+        // pca.hpp:66-109
+        if (center && nr < 1) throw std::runtime_error("cannot center with no observations");
+        if (scale && nr < 2) throw std::runtime_error("cannot scale with fewer than two observations");
+        std::shared_ptr<double> c, s;
+        if (center) c = dalloc<double>(nc);
+        if (scale) s = dalloc<double>(nc);
+        if (center || scale) m->base->colstats(center ? c.get() : nullptr, scale ? s.get() : nullptr);
+        IRLBA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        // pca.hpp:113-126: Scaled(Centered(A), column = true, divide = true).
+        std::unique_ptr<irlba_hip_matrix> chain(new irlba_hip_matrix(*m));
+        if (center) chain.reset(decorate_center(chain.get(), c));
+        if (scale) chain.reset(decorate_scale(chain.get(), s, true, true));
+        HandleOp op;
+        op.ctx = ctx;
+        op.nr = nr;
+        op.nc = nc;
+        op.m = chain.get();
+        const Opts o = to_opts(opt);
+        // Run on device buffers, then scores = U diag(d), variances = d^2/(nr-1) (pca.hpp:130-136).
+        const i64 nalloc = std::max<i64>(number, 0);
+        auto dU = dalloc<double>(nr * std::max<i64>(nalloc, 1));
+        auto dV = dalloc<double>(nc * std::max<i64>(nalloc, 1));
+        auto dD = dalloc<double>(std::max<i64>(nalloc, 1));
+        i64 nout = 0;
+        auto r = run_compute(ctx, op, number, o, dU.get(), dV.get(), dD.get(), &nout, true);
+        ctx->stats.forward_bytes = m->base->product_bytes(false);
+        ctx->stats.adjoint_bytes = m->base->product_bytes(true);
+        launch_scale_columns(dU.get(), nr, nr, static_cast<int>(nout), dD.get(), ctx->stream);
+        std::vector<double> d(nout);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(d.data(), dD.get(), sizeof(double) * nout, hipMemcpyDeviceToHost, ctx->stream));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (nc > 1) {
+            for (auto& v : d) v = v * v / static_cast<double>(nr - 1);
+        }
+        const auto kind = device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        IRLBA_HIP_CHECK(hipMemcpyAsync(scores, dU.get(), sizeof(double) * nr * nout, kind, ctx->stream));
+        IRLBA_HIP_CHECK(hipMemcpyAsync(rotation, dV.get(), sizeof(double) * nc * nout, kind, ctx->stream));
+        IRLBA_HIP_CHECK(hipMemcpyAsync(variances, d.data(), sizeof(double) * nout,
+                                       device_pointers ? hipMemcpyHostToDevice : hipMemcpyHostToHost, ctx->stream));
+        ctx->sync();
+        if (number_out) *number_out = nout;
+        if (converged) *converged = r.first;
+        if (iterations) *iterations = r.second;
+    });
+}
+
+int irlba_hip_generate_csc(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, double density, uint64_t seed,
+                           int value_type, void* values, int32_t* indices, int64_t* pointers, int64_t* nnz_out) {
+    return guarded([&] {
+        require(ctx != nullptr, "null context");
+        require(value_type == IRLBA_HIP_F32 || value_type == IRLBA_HIP_F64, "generated values are F32 or F64");
+        DeviceGuard g(ctx->device);
+        hipStream_t s = ctx->stream;
+        double t = std::floor(density * 4294967296.0);
+        t = std::min(std::max(t, 0.0), 4294967295.0);
+        const std::uint32_t thr = static_cast<std::uint32_t>(t);
+        auto counts = dalloc<i64>(ncol);
+        launch_gen_nb_count(nrow, ncol, thr, seed, counts.get(), s);
+        auto ptr = dalloc<i64>(ncol + 1);
+        exclusive_scan_i64(counts.get(), ptr.get(), ncol, s);
+        i64 nnz = 0;
+        IRLBA_HIP_CHECK(hipMemcpy(&nnz, ptr.get() + ncol, sizeof(i64), hipMemcpyDeviceToHost));
+        if (nnz_out) *nnz_out = nnz;
+        if (!values) return;
+        double table[32];
+        for (int k = 0; k < 32; ++k) table[k] = std::log1p(static_cast<double>(k));
+        auto dtab = dalloc<double>(32);
+        IRLBA_HIP_CHECK(hipMemcpy(dtab.get(), table, sizeof(table), hipMemcpyHostToDevice));
+        launch_gen_nb_fill(nrow, ncol, thr, seed, ptr.get(), dtab.get(), value_type == IRLBA_HIP_F32, values, indices,
+                           s);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(pointers, ptr.get(), sizeof(i64) * (ncol + 1), hipMemcpyDeviceToDevice, s));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int irlba_hip_generate_sparse(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, double density, uint64_t seed,
+                              int storage_type, int accumulate_f64, irlba_hip_matrix** out) {
+    return guarded([&] {
+        require(ctx && out, "null argument");
+        DeviceGuard g(ctx->device);
+        i64 nnz = 0;
+        if (irlba_hip_generate_csc(ctx, nrow, ncol, density, seed, IRLBA_HIP_F32, nullptr, nullptr, nullptr, &nnz) !=
+            IRLBA_HIP_OK)
+            throw std::runtime_error(g_last_error);
+        const int vt = storage_type == IRLBA_HIP_F64 ? IRLBA_HIP_F64 : IRLBA_HIP_F32;
+        auto vals = dalloc_bytes((vt == IRLBA_HIP_F64 ? 8 : 4) * static_cast<size_t>(nnz));
+        auto idx = dalloc<std::int32_t>(nnz);
+        auto ptr = dalloc<i64>(ncol + 1);
+        if (irlba_hip_generate_csc(ctx, nrow, ncol, density, seed, vt, vals.get(), idx.get(), ptr.get(), &nnz) !=
+            IRLBA_HIP_OK)
+            throw std::runtime_error(g_last_error);
+        auto base = make_sparse(ctx, nrow, ncol, to_storage(vt), vals.get(), idx.get(), ptr.get(), true,
+                                to_storage(storage_type), accumulate_f64 != 0, true);
+        auto* m = new irlba_hip_matrix;
+        m->ctx = ctx;
+        m->base = base;
+        *out = m;
+    });
+}
+
+}  // extern "C"

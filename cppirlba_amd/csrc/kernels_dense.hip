@@ -1,0 +1,335 @@
+// Dense mat-vec kernels for gfx950 (the SimpleMatrix path,
+// include/irlba/Matrix/simple.hpp:42-44 and :70-72).
+//
+// A dense matrix is a set of contiguous "lines" (columns if column-major).
+// One product is either
+//   * dot-lines : out[m]  = sum_j line_m[j] * xs[j]   (A'u column-major, A v row-major)
+//   * axpy-lines: out[i]  = sum_l line_l[i] * xs[l]   (A v column-major, A'u row-major)
+// Both stream the matrix once with 16-byte vector loads (2 x FP64, 4 x FP32,
+// 8 x BF16), accumulate in FP32 (FP32/BF16 storage) or FP64, and write one
+// partial per split; a reduce kernel sums the splits in a fixed order and
+// applies the fused scaling/centring/axpy epilogue in FP64.  A prologue kernel
+// scales the input vector into `xs` and computes the rank-1 dot products.
+#include "common.hpp"
+
+namespace irlba_hip {
+
+namespace {
+
+template <typename T>
+struct Vec16;  // 16-byte vector of T
+template <>
+struct Vec16<double> {
+    static constexpr int N = 2;
+};
+template <>
+struct Vec16<float> {
+    static constexpr int N = 4;
+};
+template <>
+struct Vec16<__hip_bfloat16> {
+    static constexpr int N = 8;
+};
+
+template <typename VT, typename AT, int N>
+__device__ __forceinline__ void load_vec(const VT* p, AT (&out)[N]) {
+    if constexpr (N == 1) {
+        out[0] = static_cast<AT>(to_double_any(p[0]));
+    } else if constexpr (sizeof(VT) == 8) {
+        const f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(p));
+        out[0] = v.x;
+        out[1] = v.y;
+    } else if constexpr (sizeof(VT) == 4) {
+        const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p));
+        out[0] = v.x;
+        out[1] = v.y;
+        out[2] = v.z;
+        out[3] = v.w;
+    } else {
+        const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            out[2 * k] = __uint_as_float(w[k] << 16);
+            out[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ prologue
+template <typename AT>
+__global__ void k_dense_prologue(Prologue pro, i64 n, AT* __restrict__ xs, double* __restrict__ gpart) {
+    __shared__ double red[32];
+    double g0 = 0, g1 = 0;
+    for (i64 j = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; j < n;
+         j += static_cast<i64>(gridDim.x) * blockDim.x) {
+        const double xv = pro.x[j];
+        xs[j] = static_cast<AT>(pro.xscale ? xv * pro.xscale[j] : xv);
+        if (pro.n_terms > 0) g0 += (pro.tdot[0] ? pro.tdot[0][j] : 1.0) * xv;
+        if (pro.n_terms > 1) g1 += (pro.tdot[1] ? pro.tdot[1][j] : 1.0) * xv;
+    }
+    block_sum2(g0, g1, red);
+    if (threadIdx.x == 0) {
+        gpart[2 * blockIdx.x] = g0;
+        gpart[2 * blockIdx.x + 1] = g1;
+    }
+}
+
+// ------------------------------------------------------------------ dot-lines
+// Block: R lines x one segment of SEG elements; 256 threads, N elements each per step.
+template <typename VT, typename AT, int N, int R>
+__global__ __launch_bounds__(256) void k_dot_lines(const VT* __restrict__ A, i64 n_lines, i64 Ln, i64 seg_len,
+                                                   const AT* __restrict__ xs, AT* __restrict__ partial) {
+    __shared__ AT red[R][4];
+    const i64 m0 = static_cast<i64>(blockIdx.x) * R;
+    const i64 j0 = static_cast<i64>(blockIdx.y) * seg_len;
+    const i64 j1 = min(Ln, j0 + seg_len);
+    AT acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0;
+    for (i64 j = j0 + static_cast<i64>(threadIdx.x) * N; j < j1; j += 256 * N) {
+        AT x[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) x[k] = xs[j + k];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (m0 + r < n_lines) {
+                AT a[N];
+                load_vec<VT, AT, N>(A + (m0 + r) * Ln + j, a);
+#pragma unroll
+                for (int k = 0; k < N; ++k) acc[r] += a[k] * x[k];
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const AT v = wave_sum(acc[r]);
+        if (lane == 0) red[r][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < R && m0 + threadIdx.x < n_lines) {
+        const int r = threadIdx.x;
+        partial[static_cast<i64>(blockIdx.y) * n_lines + m0 + r] = red[r][0] + red[r][1] + red[r][2] + red[r][3];
+    }
+}
+
+// ------------------------------------------------------------------ axpy-lines
+// Block: 256 threads x N outputs each; grid.y splits the lines into chunks.
+template <typename VT, typename AT, int N>
+__global__ __launch_bounds__(256) void k_axpy_lines(const VT* __restrict__ A, i64 n_lines, i64 Ln, i64 chunk,
+                                                    const AT* __restrict__ xs, AT* __restrict__ partial) {
+    const i64 i = (static_cast<i64>(blockIdx.x) * 256 + threadIdx.x) * N;
+    const i64 l0 = static_cast<i64>(blockIdx.y) * chunk;
+    const i64 l1 = min(n_lines, l0 + chunk);
+    if (i >= Ln) return;
+    AT acc[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) acc[k] = 0;
+    if (i + N <= Ln) {
+        i64 l = l0;
+        for (; l + 3 < l1; l += 4) {
+            AT a[4][N];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) load_vec<VT, AT, N>(A + (l + u) * Ln + i, a[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const AT xv = xs[l + u];
+#pragma unroll
+                for (int k = 0; k < N; ++k) acc[k] += a[u][k] * xv;
+            }
+        }
+        for (; l < l1; ++l) {
+            AT a[N];
+            load_vec<VT, AT, N>(A + l * Ln + i, a);
+            const AT xv = xs[l];
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc[k] += a[k] * xv;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) partial[static_cast<i64>(blockIdx.y) * Ln + i + k] = acc[k];
+    } else {
+        for (int k = 0; k < N && i + k < Ln; ++k) {
+            AT s = 0;
+            for (i64 l = l0; l < l1; ++l) s += static_cast<AT>(to_double_any(A[l * Ln + i + k])) * xs[l];
+            partial[static_cast<i64>(blockIdx.y) * Ln + i + k] = s;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ reduce + epilogue
+template <typename AT>
+__global__ void k_dense_reduce(const AT* __restrict__ partial, int splits, i64 n_out, const double* __restrict__ gpart,
+                               int n_gpart, Epilogue epi) {
+    double gamma[2] = {0, 0};
+    for (int b = 0; b < n_gpart; ++b) {
+        gamma[0] += gpart[2 * b];
+        gamma[1] += gpart[2 * b + 1];
+    }
+    for (i64 m = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; m < n_out;
+         m += static_cast<i64>(gridDim.x) * blockDim.x) {
+        double acc = 0;
+        for (int s = 0; s < splits; ++s) acc += static_cast<double>(partial[static_cast<i64>(s) * n_out + m]);
+        epi.out[m] = apply_epilogue(epi, m, acc, gamma);
+    }
+}
+
+template <typename VT, typename AT>
+void dense_matvec_typed(const DenseDesc& d, bool adjoint, const Prologue& pro, const Epilogue& epi, void* xs_scratch,
+                        double* gamma_scratch, void* partial_v, i64 partial_cap, hipStream_t s, hipEvent_t ev0,
+                        hipEvent_t ev1) {
+    const VT* A = static_cast<const VT*>(d.a);
+    AT* xs = static_cast<AT*>(xs_scratch);
+    AT* partial = static_cast<AT*>(partial_v);
+    const i64 n_lines = d.col_major ? d.nc : d.nr;
+    const i64 Ln = d.col_major ? d.nr : d.nc;
+    // Input length and whether the product is a dot over lines.
+    const bool dot = d.col_major ? adjoint : !adjoint;
+    const i64 n_in = dot ? Ln : n_lines;
+    const i64 n_out = dot ? n_lines : Ln;
+
+    const int pro_blocks = static_cast<int>(std::max<i64>(1, std::min<i64>((n_in + 255) / 256, 256)));
+    hipLaunchKernelGGL(k_dense_prologue<AT>, dim3(pro_blocks), dim3(256), 0, s, pro, n_in, xs, gamma_scratch);
+    IRLBA_HIP_LAUNCH_CHECK();
+
+    constexpr int NV = Vec16<VT>::N;
+    int splits = 1;
+    if (dot) {
+        constexpr int R = 4;
+        const i64 groups = (n_lines + R - 1) / R;
+        // Segment the lines so that there are enough blocks to fill the chip.
+        i64 nseg = std::max<i64>(1, std::min<i64>(1024 / std::max<i64>(groups, 1), (Ln + 8191) / 8192));
+        i64 seg_len = (Ln + nseg - 1) / nseg;
+        const bool vec_ok = (Ln % NV) == 0;
+        if (vec_ok) seg_len = (seg_len + NV - 1) / NV * NV;
+        nseg = (Ln + seg_len - 1) / seg_len;
+        if (nseg * n_lines > partial_cap) throw std::runtime_error("dense partial buffer too small");
+        const dim3 g(static_cast<unsigned>(groups), static_cast<unsigned>(nseg));
+        if (ev0) IRLBA_HIP_CHECK(hipEventRecord(ev0, s));
+        if (vec_ok)
+            hipLaunchKernelGGL((k_dot_lines<VT, AT, NV, R>), g, dim3(256), 0, s, A, n_lines, Ln, seg_len, xs, partial);
+        else
+            hipLaunchKernelGGL((k_dot_lines<VT, AT, 1, R>), g, dim3(256), 0, s, A, n_lines, Ln, seg_len, xs, partial);
+        IRLBA_HIP_LAUNCH_CHECK();
+        if (ev1) IRLBA_HIP_CHECK(hipEventRecord(ev1, s));
+        splits = static_cast<int>(nseg);
+    } else {
+        const bool vec_ok = (Ln % NV) == 0;
+        const int N = vec_ok ? NV : 1;
+        const i64 xblocks = (Ln + 256 * N - 1) / (256 * N);
+        i64 nchunk = std::max<i64>(1, std::min<i64>(std::max<i64>(1, 2048 / xblocks), (n_lines + 63) / 64));
+        const i64 chunk = (n_lines + nchunk - 1) / nchunk;
+        nchunk = (n_lines + chunk - 1) / chunk;
+        if (nchunk * Ln > partial_cap) throw std::runtime_error("dense partial buffer too small");
+        const dim3 g(static_cast<unsigned>(xblocks), static_cast<unsigned>(nchunk));
+        if (ev0) IRLBA_HIP_CHECK(hipEventRecord(ev0, s));
+        if (vec_ok)
+            hipLaunchKernelGGL((k_axpy_lines<VT, AT, NV>), g, dim3(256), 0, s, A, n_lines, Ln, chunk, xs, partial);
+        else
+            hipLaunchKernelGGL((k_axpy_lines<VT, AT, 1>), g, dim3(256), 0, s, A, n_lines, Ln, chunk, xs, partial);
+        IRLBA_HIP_LAUNCH_CHECK();
+        if (ev1) IRLBA_HIP_CHECK(hipEventRecord(ev1, s));
+        splits = static_cast<int>(nchunk);
+    }
+    const i64 rgrid = std::max<i64>(1, std::min<i64>((n_out + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_dense_reduce<AT>, dim3(static_cast<unsigned>(rgrid)), dim3(256), 0, s,
+                       static_cast<const AT*>(partial), splits, n_out, gamma_scratch, pro_blocks, epi);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ realize / column statistics
+template <typename VT>
+__global__ void k_dense_realize(const VT* __restrict__ A, i64 nr, i64 nc, int col_major, double* __restrict__ out) {
+    const i64 n = nr * nc;
+    for (i64 e = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<i64>(gridDim.x) * blockDim.x) {
+        const i64 r = e % nr, c = e / nr;  // column-major output position
+        out[e] = to_double_any(col_major ? A[r + c * nr] : A[r * nc + c]);
+    }
+}
+
+// Column mean / SD as in pca.hpp:85-109 (two passes, all rows).
+template <typename VT>
+__global__ void k_dense_colstats(const VT* __restrict__ A, i64 nr, i64 nc, int col_major, double* center_out,
+                                 double* scale_out) {
+    __shared__ double red[32];
+    for (i64 c = blockIdx.x; c < nc; c += gridDim.x) {
+        double sum = 0, dummy = 0;
+        for (i64 r = threadIdx.x; r < nr; r += blockDim.x)
+            sum += to_double_any(col_major ? A[r + c * nr] : A[r * nc + c]);
+        block_sum2(sum, dummy, red);
+        const double mean = center_out ? sum / nr : 0.0;
+        if (center_out && threadIdx.x == 0) center_out[c] = mean;
+        if (scale_out) {
+            double ss = 0;
+            dummy = 0;
+            for (i64 r = threadIdx.x; r < nr; r += blockDim.x) {
+                const double dlt = to_double_any(col_major ? A[r + c * nr] : A[r * nc + c]) - mean;
+                ss += dlt * dlt;
+            }
+            block_sum2(ss, dummy, red);
+            if (threadIdx.x == 0) scale_out[c] = ss != 0 ? sqrt(ss / static_cast<double>(nr - 1)) : 1.0;
+        }
+    }
+}
+
+}  // namespace
+
+void launch_dense_matvec(const DenseDesc& d, bool adjoint, const Prologue& pro, const Epilogue& epi, void* xs_scratch,
+                         double* gamma_scratch, void* partial, i64 partial_cap, hipStream_t s, hipEvent_t ev0,
+                         hipEvent_t ev1) {
+    if (d.nr == 0 || d.nc == 0) return;
+    if (d.st == ST_F64)
+        dense_matvec_typed<double, double>(d, adjoint, pro, epi, xs_scratch, gamma_scratch, partial, partial_cap, s,
+                                           ev0, ev1);
+    else if (d.st == ST_F32)
+        dense_matvec_typed<float, float>(d, adjoint, pro, epi, xs_scratch, gamma_scratch, partial, partial_cap, s,
+                                         ev0, ev1);
+    else
+        dense_matvec_typed<__hip_bfloat16, float>(d, adjoint, pro, epi, xs_scratch, gamma_scratch, partial,
+                                                  partial_cap, s, ev0, ev1);
+}
+
+void launch_dense_realize(const DenseDesc& d, double* dense, hipStream_t s) {
+    const i64 n = d.nr * d.nc;
+    if (n == 0) return;
+    const i64 grid = std::max<i64>(1, std::min<i64>((n + 255) / 256, 65536));
+    const dim3 g(static_cast<unsigned>(grid)), b(256);
+    const int cm = d.col_major ? 1 : 0;
+    if (d.st == ST_F64)
+        hipLaunchKernelGGL(k_dense_realize<double>, g, b, 0, s, static_cast<const double*>(d.a), d.nr, d.nc, cm, dense);
+    else if (d.st == ST_F32)
+        hipLaunchKernelGGL(k_dense_realize<float>, g, b, 0, s, static_cast<const float*>(d.a), d.nr, d.nc, cm, dense);
+    else
+        hipLaunchKernelGGL(k_dense_realize<__hip_bfloat16>, g, b, 0, s, static_cast<const __hip_bfloat16*>(d.a), d.nr,
+                           d.nc, cm, dense);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+void launch_dense_colstats(const DenseDesc& d, double* center_out, double* scale_out, hipStream_t s) {
+    if (d.nc == 0) return;
+    const i64 grid = std::max<i64>(1, std::min<i64>(d.nc, 65536));
+    const dim3 g(static_cast<unsigned>(grid)), b(256);
+    const int cm = d.col_major ? 1 : 0;
+    if (d.st == ST_F64)
+        hipLaunchKernelGGL(k_dense_colstats<double>, g, b, 0, s, static_cast<const double*>(d.a), d.nr, d.nc, cm,
+                           center_out, scale_out);
+    else if (d.st == ST_F32)
+        hipLaunchKernelGGL(k_dense_colstats<float>, g, b, 0, s, static_cast<const float*>(d.a), d.nr, d.nc, cm,
+                           center_out, scale_out);
+    else
+        hipLaunchKernelGGL(k_dense_colstats<__hip_bfloat16>, g, b, 0, s, static_cast<const __hip_bfloat16*>(d.a), d.nr,
+                           d.nc, cm, center_out, scale_out);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+i64 dense_partial_elems(const DenseDesc& d) {
+    // Upper bound on the partial buffer used by either direction.
+    const i64 n_lines = d.col_major ? d.nc : d.nr;
+    const i64 Ln = d.col_major ? d.nr : d.nc;
+    const i64 dot_bound = n_lines * std::max<i64>(1, (Ln + 8191) / 8192 + 1);
+    const i64 axpy_bound = Ln * std::max<i64>(1, (n_lines + 63) / 64 + 1);
+    return std::max(dot_bound, axpy_bound);
+}
+
+}  // namespace irlba_hip

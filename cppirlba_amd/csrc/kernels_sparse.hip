@@ -1,0 +1,615 @@
+// Sparse kernels for gfx950: device-side construction of the tiled line
+// layouts (both orientations of a CSC/CSR matrix) and the tiled SpMV that
+// serves both A.v and A'.u.
+//
+// The reference's ParallelSparseMatrix (include/irlba/Matrix/sparse.hpp:185-267)
+// computes CSC A.v as a scatter into thread-private row ranges and CSC A'.u as a
+// per-column gather.  Here both directions are a gather over "lines": the
+// matrix is kept twice in HBM, once per orientation, each split along its input
+// (minor) dimension into tiles that fit LDS.  A workgroup stages one tile of
+// the input vector in LDS (fusing the column/row scaling and the rank-1
+// centring dot products into that pass), then streams values and 16-bit
+// in-tile indices of its nnz-balanced chunk of lines with non-temporal vector
+// loads, one group of lanes per line, and writes either the finished output
+// (single tile: fused epilogue) or one FP partial per (tile, line).
+#include "common.hpp"
+
+namespace irlba_hip {
+
+// ------------------------------------------------------------------ build: count / scatter
+__global__ void k_count_entries(const std::int32_t* __restrict__ idx, const i64* __restrict__ ptr, i64 n_primary,
+                                int transpose, int tile_len, i64 n_major_out, unsigned long long* counts) {
+    const int lane = threadIdx.x & 63;
+    const i64 wave = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const i64 nwaves = (static_cast<i64>(gridDim.x) * blockDim.x) >> 6;
+    for (i64 p = wave; p < n_primary; p += nwaves) {
+        const i64 e0 = ptr[p], e1 = ptr[p + 1];
+        for (i64 e = e0 + lane; e < e1; e += 64) {
+            const i64 s = idx[e];
+            const i64 major = transpose ? s : p;
+            const i64 minor = transpose ? p : s;
+            const i64 t = minor / tile_len;
+            atomicAdd(&counts[t * n_major_out + major], 1ULL);
+        }
+    }
+}
+
+void launch_count_entries(const std::int32_t* idx, const i64* ptr, i64 n_primary, bool transpose, int tile_len,
+                          i64 n_major_out, unsigned long long* counts, hipStream_t s) {
+    const int block = 256;
+    const i64 waves_needed = n_primary;
+    const i64 grid = std::max<i64>(1, std::min<i64>((waves_needed * 64 + block - 1) / block, 65536));
+    hipLaunchKernelGGL(k_count_entries, dim3(static_cast<unsigned>(grid)), dim3(block), 0, s, idx, ptr, n_primary,
+                       transpose ? 1 : 0, tile_len, n_major_out, counts);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+__global__ void k_pad_counts(const unsigned long long* __restrict__ c, i64* __restrict__ plen, i64 n, int vec) {
+    for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * blockDim.x) {
+        const i64 v = static_cast<i64>(c[i]);
+        plen[i] = (v + vec - 1) / vec * vec;
+    }
+}
+
+void launch_pad_counts(const unsigned long long* counts, i64* plen, i64 n, int vec, hipStream_t s) {
+    const i64 grid = std::max<i64>(1, std::min<i64>((n + 255) / 256, 65536));
+    hipLaunchKernelGGL(k_pad_counts, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, counts, plen, n, vec);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ exclusive scan (int64)
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__global__ void k_scan_block_sums(const i64* __restrict__ in, i64 n, i64* __restrict__ sums) {
+    __shared__ i64 red[kScanThreads / 64];
+    const i64 base = static_cast<i64>(blockIdx.x) * kScanTile;
+    i64 acc = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const i64 i = base + k * kScanThreads + threadIdx.x;
+        if (i < n) acc += in[i];
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        i64 t = 0;
+        for (int w = 0; w < kScanThreads / 64; ++w) t += red[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+__global__ void k_scan_apply(const i64* __restrict__ in, i64* __restrict__ out, i64 n,
+                             const i64* __restrict__ block_offsets) {
+    __shared__ i64 thread_tot[kScanThreads];
+    const i64 base = static_cast<i64>(blockIdx.x) * kScanTile + static_cast<i64>(threadIdx.x) * kScanItems;
+    i64 vals[kScanItems];
+    i64 tot = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const i64 i = base + k;
+        vals[k] = i < n ? in[i] : 0;
+        tot += vals[k];
+    }
+    thread_tot[threadIdx.x] = tot;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over the 256 thread totals.
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const i64 v = threadIdx.x >= static_cast<unsigned>(off) ? thread_tot[threadIdx.x - off] : 0;
+        __syncthreads();
+        thread_tot[threadIdx.x] += v;
+        __syncthreads();
+    }
+    i64 run = block_offsets[blockIdx.x] + (threadIdx.x ? thread_tot[threadIdx.x - 1] : 0);
+    for (int k = 0; k < kScanItems; ++k) {
+        const i64 i = base + k;
+        if (i < n) out[i] = run;
+        run += vals[k];
+        if (i == n - 1) out[n] = run;
+    }
+}
+
+void exclusive_scan_i64(const i64* in, i64* out, i64 n, hipStream_t s) {
+    if (n <= 0) {
+        const i64 zero = 0;
+        IRLBA_HIP_CHECK(hipMemcpyAsync(out, &zero, sizeof(i64), hipMemcpyHostToDevice, s));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+        return;
+    }
+    const i64 nb = (n + kScanTile - 1) / kScanTile;
+    i64* sums = nullptr;
+    i64* offs = nullptr;
+    IRLBA_HIP_CHECK(hipMalloc(&sums, sizeof(i64) * nb));
+    IRLBA_HIP_CHECK(hipMalloc(&offs, sizeof(i64) * (nb + 1)));
+    hipLaunchKernelGGL(k_scan_block_sums, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s, in, n, sums);
+    IRLBA_HIP_LAUNCH_CHECK();
+    if (nb == 1) {
+        const i64 zero = 0;
+        IRLBA_HIP_CHECK(hipMemcpyAsync(offs, &zero, sizeof(i64), hipMemcpyHostToDevice, s));
+    } else {
+        exclusive_scan_i64(sums, offs, nb, s);
+    }
+    hipLaunchKernelGGL(k_scan_apply, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s, in, out, n, offs);
+    IRLBA_HIP_LAUNCH_CHECK();
+    IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+    IRLBA_HIP_CHECK(hipFree(sums));
+    IRLBA_HIP_CHECK(hipFree(offs));
+}
+
+// ------------------------------------------------------------------ scatter into (tile, line) buckets
+template <typename TIn, typename TOut>
+__device__ __forceinline__ TOut convert_value(TIn v) {
+    return static_cast<TOut>(v);
+}
+template <>
+__device__ __forceinline__ __hip_bfloat16 convert_value<double, __hip_bfloat16>(double v) {
+    return __float2bfloat16(static_cast<float>(v));
+}
+template <>
+__device__ __forceinline__ __hip_bfloat16 convert_value<float, __hip_bfloat16>(float v) {
+    return __float2bfloat16(v);
+}
+
+template <typename TIn, typename TOut>
+__global__ void k_scatter_entries(const std::int32_t* __restrict__ idx, const TIn* __restrict__ val,
+                                  const i64* __restrict__ ptr, i64 n_primary, int transpose, int tile_len,
+                                  i64 n_major_out, unsigned long long* cursor, std::uint16_t* __restrict__ out_idx,
+                                  TOut* __restrict__ out_val) {
+    const int lane = threadIdx.x & 63;
+    const i64 wave = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const i64 nwaves = (static_cast<i64>(gridDim.x) * blockDim.x) >> 6;
+    for (i64 p = wave; p < n_primary; p += nwaves) {
+        const i64 e0 = ptr[p], e1 = ptr[p + 1];
+        for (i64 e = e0 + lane; e < e1; e += 64) {
+            const i64 sidx = idx[e];
+            const i64 major = transpose ? sidx : p;
+            const i64 minor = transpose ? p : sidx;
+            const i64 t = minor / tile_len;
+            const i64 pos = static_cast<i64>(atomicAdd(&cursor[t * n_major_out + major], 1ULL));
+            out_idx[pos] = static_cast<std::uint16_t>(minor - t * tile_len);
+            out_val[pos] = convert_value<TIn, TOut>(val[e]);
+        }
+    }
+}
+
+void launch_scatter_entries(const std::int32_t* idx, const void* val_in, StorageType in_t, const i64* ptr,
+                            i64 n_primary, bool transpose, int tile_len, i64 n_major_out,
+                            unsigned long long* cursor, std::uint16_t* out_idx, void* out_val, StorageType out_t,
+                            hipStream_t s) {
+    const int block = 256;
+    const i64 grid = std::max<i64>(1, std::min<i64>((n_primary * 64 + block - 1) / block, 65536));
+    const dim3 g(static_cast<unsigned>(grid)), b(block);
+    const int tr = transpose ? 1 : 0;
+#define IRLBA_SCATTER(TI, TO)                                                                                  \
+    hipLaunchKernelGGL((k_scatter_entries<TI, TO>), g, b, 0, s, idx, static_cast<const TI*>(val_in), ptr,      \
+                       n_primary, tr, tile_len, n_major_out, cursor, out_idx, static_cast<TO*>(out_val))
+    if (in_t == ST_F64 && out_t == ST_F64) IRLBA_SCATTER(double, double);
+    else if (in_t == ST_F64 && out_t == ST_F32) IRLBA_SCATTER(double, float);
+    else if (in_t == ST_F32 && out_t == ST_F32) IRLBA_SCATTER(float, float);
+    else if (in_t == ST_F32 && out_t == ST_F64) IRLBA_SCATTER(float, double);
+    else if (in_t == ST_F64 && out_t == ST_BF16) IRLBA_SCATTER(double, __hip_bfloat16);
+    else if (in_t == ST_F32 && out_t == ST_BF16) IRLBA_SCATTER(float, __hip_bfloat16);
+    else throw std::runtime_error("unsupported sparse value conversion");
+#undef IRLBA_SCATTER
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ segment sort (bitmap ranks)
+// Entries of one (tile, line) segment have distinct 16-bit keys below tile_len.
+// Each key sets one bit of an LDS bitmap; its rank is the popcount of the bits
+// below it, so the sorted position needs no comparisons and no global order.
+template <typename V>
+__global__ __launch_bounds__(256) void k_sort_segments(const i64* __restrict__ ptr,
+                                                       const unsigned long long* __restrict__ counts, i64 n_seg,
+                                                       int words, const std::uint16_t* __restrict__ in_idx,
+                                                       const V* __restrict__ in_val, std::uint16_t* __restrict__ out_idx,
+                                                       V* __restrict__ out_val, int* dup_flag) {
+    __shared__ unsigned int bitmap[2048];
+    __shared__ int prefix[2048];
+    __shared__ int thread_tot[256];
+    for (i64 seg = blockIdx.x; seg < n_seg; seg += gridDim.x) {
+        const i64 start = ptr[seg];
+        const int n = static_cast<int>(counts[seg]);
+        if (n <= 1) {
+            if (n == 1 && threadIdx.x == 0) {
+                out_idx[start] = in_idx[start];
+                out_val[start] = in_val[start];
+            }
+            continue;
+        }
+        for (int w = threadIdx.x; w < words; w += blockDim.x) bitmap[w] = 0u;
+        __syncthreads();
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const unsigned key = in_idx[start + e];
+            atomicOr(&bitmap[key >> 5], 1u << (key & 31));
+        }
+        __syncthreads();
+        const int per = (words + blockDim.x - 1) / blockDim.x;
+        const int w0 = threadIdx.x * per;
+        int tot = 0;
+        for (int w = w0; w < w0 + per && w < words; ++w) tot += __popc(bitmap[w]);
+        thread_tot[threadIdx.x] = tot;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int v = threadIdx.x >= static_cast<unsigned>(off) ? thread_tot[threadIdx.x - off] : 0;
+            __syncthreads();
+            thread_tot[threadIdx.x] += v;
+            __syncthreads();
+        }
+        int run = threadIdx.x ? thread_tot[threadIdx.x - 1] : 0;
+        for (int w = w0; w < w0 + per && w < words; ++w) {
+            prefix[w] = run;
+            run += __popc(bitmap[w]);
+        }
+        if (threadIdx.x == 255 && thread_tot[255] != n) atomicExch(dup_flag, 1);
+        __syncthreads();
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const unsigned key = in_idx[start + e];
+            const unsigned below = bitmap[key >> 5] & ((1u << (key & 31)) - 1u);
+            const int rank = prefix[key >> 5] + __popc(below);
+            if (rank < n) {
+                out_idx[start + rank] = static_cast<std::uint16_t>(key);
+                out_val[start + rank] = in_val[start + e];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+void launch_sort_segments(const i64* ptr, const unsigned long long* counts, i64 n_seg, int tile_len,
+                          const std::uint16_t* in_idx, const void* in_val, std::uint16_t* out_idx, void* out_val,
+                          StorageType st, int* dup_flag, hipStream_t s) {
+    const int words = (tile_len + 31) / 32;
+    if (words > 2048) throw std::runtime_error("tile too long for the segment sort");
+    const i64 grid = std::max<i64>(1, std::min<i64>(n_seg, 1 << 20));
+    const dim3 g(static_cast<unsigned>(grid)), b(256);
+    if (st == ST_F64)
+        hipLaunchKernelGGL(k_sort_segments<double>, g, b, 0, s, ptr, counts, n_seg, words, in_idx,
+                           static_cast<const double*>(in_val), out_idx, static_cast<double*>(out_val), dup_flag);
+    else if (st == ST_F32)
+        hipLaunchKernelGGL(k_sort_segments<float>, g, b, 0, s, ptr, counts, n_seg, words, in_idx,
+                           static_cast<const float*>(in_val), out_idx, static_cast<float*>(out_val), dup_flag);
+    else
+        hipLaunchKernelGGL(k_sort_segments<__hip_bfloat16>, g, b, 0, s, ptr, counts, n_seg, words, in_idx,
+                           static_cast<const __hip_bfloat16*>(in_val), out_idx,
+                           static_cast<__hip_bfloat16*>(out_val), dup_flag);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ nnz-balanced chunks
+__global__ void k_chunk_bounds(const i64* __restrict__ ptr, i64 n_major, int n_tiles, int G, int* chunk) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n_tiles * (G + 1)) return;
+    const int t = id / (G + 1), g = id % (G + 1);
+    const i64* P = ptr + static_cast<i64>(t) * n_major;
+    const i64 total = P[n_major] - P[0];
+    const i64 target = P[0] + (g == G ? total : (total / G) * g + ((total % G) * g) / G);
+    i64 lo = 0, hi = n_major;  // smallest m with P[m] >= target
+    while (lo < hi) {
+        const i64 mid = (lo + hi) / 2;
+        if (P[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    chunk[id] = static_cast<int>(g == G ? n_major : (g == 0 ? 0 : lo));
+}
+
+void launch_chunk_bounds(const i64* ptr, i64 n_major, int n_tiles, int G, int* chunk, hipStream_t s) {
+    const int n = n_tiles * (G + 1);
+    hipLaunchKernelGGL(k_chunk_bounds, dim3((n + 255) / 256), dim3(256), 0, s, ptr, n_major, n_tiles, G, chunk);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ tiled SpMV
+template <typename AT>
+struct SpmvParams {
+    const i64* ptr;
+    const std::uint16_t* idx;
+    const void* val;
+    const int* chunk;
+    i64 n_major, n_minor;
+    int tile_len, n_tiles, G;
+    Prologue pro;
+    Epilogue epi;
+    AT* partial;
+    double* gamma_part;
+};
+
+template <typename VT, typename AT, int VEC>
+struct Loader;
+
+template <typename AT>
+struct Loader<float, AT, 4> {
+    __device__ __forceinline__ static void load(const float* v, const std::uint16_t* ix, i64 s, float (&vo)[4],
+                                                unsigned (&io)[4]) {
+        const f32x4 f = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(v + s));
+        const u32x2 q = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(ix + s));
+        vo[0] = f.x; vo[1] = f.y; vo[2] = f.z; vo[3] = f.w;
+        io[0] = q.x & 0xffffu; io[1] = q.x >> 16; io[2] = q.y & 0xffffu; io[3] = q.y >> 16;
+    }
+};
+template <typename AT>
+struct Loader<double, AT, 4> {
+    __device__ __forceinline__ static void load(const double* v, const std::uint16_t* ix, i64 s, double (&vo)[4],
+                                                unsigned (&io)[4]) {
+        const f64x2 a = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(v + s));
+        const f64x2 b = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(v + s + 2));
+        const u32x2 q = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(ix + s));
+        vo[0] = a.x; vo[1] = a.y; vo[2] = b.x; vo[3] = b.y;
+        io[0] = q.x & 0xffffu; io[1] = q.x >> 16; io[2] = q.y & 0xffffu; io[3] = q.y >> 16;
+    }
+};
+template <typename VT, typename AT>
+struct Loader<VT, AT, 1> {
+    __device__ __forceinline__ static void load(const VT* v, const std::uint16_t* ix, i64 s, VT (&vo)[1],
+                                                unsigned (&io)[1]) {
+        vo[0] = __builtin_nontemporal_load(v + s);
+        io[0] = __builtin_nontemporal_load(ix + s);
+    }
+};
+
+template <typename VT, typename AT, int LPL, int VEC>
+__global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
+    extern __shared__ unsigned char smem_raw[];
+    AT* xs = reinterpret_cast<AT*>(smem_raw);
+    __shared__ double red[2 * 16];
+
+    const int t = blockIdx.x / p.G;
+    const int g = blockIdx.x - t * p.G;
+    const i64 base = static_cast<i64>(t) * p.tile_len;
+    const int cnt = static_cast<int>(min(static_cast<i64>(p.tile_len), p.n_minor - base));
+
+    // Stage the input tile (scaled) in LDS; accumulate the rank-1 dot products.
+    double g0 = 0, g1 = 0;
+    const Prologue& pro = p.pro;
+    for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+        const double xv = pro.x[base + j];
+        const double xsv = pro.xscale ? xv * pro.xscale[base + j] : xv;
+        xs[j] = static_cast<AT>(xsv);
+        if (pro.n_terms > 0) g0 += (pro.tdot[0] ? pro.tdot[0][base + j] : 1.0) * xv;
+        if (pro.n_terms > 1) g1 += (pro.tdot[1] ? pro.tdot[1][base + j] : 1.0) * xv;
+    }
+    if (pro.n_terms > 0) {
+        block_sum2(g0, g1, red);
+    } else {
+        __syncthreads();
+    }
+    if (p.n_tiles > 1 && g == 0 && threadIdx.x == 0) {
+        p.gamma_part[2 * t] = g0;
+        p.gamma_part[2 * t + 1] = g1;
+    }
+    const double gamma[2] = {g0, g1};
+
+    const VT* __restrict__ val = static_cast<const VT*>(p.val);
+    const std::uint16_t* __restrict__ ix = p.idx;
+    const i64* __restrict__ P = p.ptr + static_cast<i64>(t) * p.n_major;
+    const int L0 = p.chunk[t * (p.G + 1) + g], L1 = p.chunk[t * (p.G + 1) + g + 1];
+    const int lane_in = threadIdx.x & (LPL - 1);
+    const int grp = threadIdx.x / LPL, ngrp = blockDim.x / LPL;
+    constexpr int STRIDE = LPL * VEC;
+
+    for (int m = L0 + grp; m < L1; m += ngrp) {
+        const i64 s0 = P[m], s1 = P[m + 1];
+        AT acc0 = 0, acc1 = 0;
+        i64 s = s0 + lane_in * VEC;
+        for (; s + 3 * STRIDE < s1; s += 4 * STRIDE) {
+            VT v[4][VEC];
+            unsigned q[4][VEC];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) Loader<VT, AT, VEC>::load(val, ix, s + u * STRIDE, v[u], q[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int k = 0; k < VEC; ++k) {
+                    if (u & 1) acc1 += static_cast<AT>(v[u][k]) * xs[q[u][k]];
+                    else acc0 += static_cast<AT>(v[u][k]) * xs[q[u][k]];
+                }
+        }
+        for (; s < s1; s += STRIDE) {
+            VT v[VEC];
+            unsigned q[VEC];
+            Loader<VT, AT, VEC>::load(val, ix, s, v, q);
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) acc0 += static_cast<AT>(v[k]) * xs[q[k]];
+        }
+        AT acc = group_sum<LPL>(acc0 + acc1);
+        if (lane_in == 0) {
+            if (p.n_tiles == 1) p.epi.out[m] = apply_epilogue(p.epi, m, static_cast<double>(acc), gamma);
+            else p.partial[static_cast<i64>(t) * p.n_major + m] = acc;
+        }
+    }
+}
+
+template <typename AT>
+__global__ void k_spmv_reduce(const AT* __restrict__ partial, int n_tiles, i64 n_major,
+                              const double* __restrict__ gamma_part, Epilogue epi) {
+    double gamma[2] = {0, 0};
+    for (int t = 0; t < n_tiles; ++t) {
+        gamma[0] += gamma_part[2 * t];
+        gamma[1] += gamma_part[2 * t + 1];
+    }
+    for (i64 m = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; m < n_major;
+         m += static_cast<i64>(gridDim.x) * blockDim.x) {
+        double acc = 0;
+        for (int t = 0; t < n_tiles; ++t) acc += static_cast<double>(partial[static_cast<i64>(t) * n_major + m]);
+        epi.out[m] = apply_epilogue(epi, m, acc, gamma);
+    }
+}
+
+template <typename VT, typename AT, int LPL, int VEC>
+static void spmv_launch_one(const TiledLines& tl, const Prologue& pro, const Epilogue& epi, void* partial,
+                            double* gamma_part, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    SpmvParams<AT> p;
+    p.ptr = tl.ptr;
+    p.idx = tl.idx;
+    p.val = tl.val;
+    p.chunk = tl.chunk;
+    p.n_major = tl.n_major;
+    p.n_minor = tl.n_minor;
+    p.tile_len = tl.tile_len;
+    p.n_tiles = tl.n_tiles;
+    p.G = tl.G;
+    p.pro = pro;
+    p.epi = epi;
+    p.partial = static_cast<AT*>(partial);
+    p.gamma_part = gamma_part;
+    const size_t lds = static_cast<size_t>(tl.tile_len) * sizeof(AT);
+    if (ev0) IRLBA_HIP_CHECK(hipEventRecord(ev0, s));
+    hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, VEC>), dim3(tl.G * tl.n_tiles), dim3(1024), lds, s, p);
+    IRLBA_HIP_LAUNCH_CHECK();
+    if (ev1) IRLBA_HIP_CHECK(hipEventRecord(ev1, s));
+    if (tl.n_tiles > 1) {
+        const i64 grid = std::max<i64>(1, std::min<i64>((tl.n_major + 255) / 256, 8192));
+        hipLaunchKernelGGL(k_spmv_reduce<AT>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s,
+                           static_cast<const AT*>(partial), tl.n_tiles, tl.n_major, gamma_part, epi);
+        IRLBA_HIP_LAUNCH_CHECK();
+    }
+}
+
+template <typename VT, typename AT, int VEC>
+static void spmv_dispatch_lpl(const TiledLines& tl, const Prologue& pro, const Epilogue& epi, void* partial,
+                              double* gamma_part, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    switch (tl.lpl) {
+        case 4: spmv_launch_one<VT, AT, 4, VEC>(tl, pro, epi, partial, gamma_part, s, ev0, ev1); break;
+        case 8: spmv_launch_one<VT, AT, 8, VEC>(tl, pro, epi, partial, gamma_part, s, ev0, ev1); break;
+        case 16: spmv_launch_one<VT, AT, 16, VEC>(tl, pro, epi, partial, gamma_part, s, ev0, ev1); break;
+        case 32: spmv_launch_one<VT, AT, 32, VEC>(tl, pro, epi, partial, gamma_part, s, ev0, ev1); break;
+        default: spmv_launch_one<VT, AT, 64, VEC>(tl, pro, epi, partial, gamma_part, s, ev0, ev1); break;
+    }
+}
+
+template <typename VT, typename AT>
+static void spmv_dispatch_vec(const TiledLines& tl, const Prologue& pro, const Epilogue& epi, void* partial,
+                              double* gamma_part, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    if (tl.vec == 4) spmv_dispatch_lpl<VT, AT, 4>(tl, pro, epi, partial, gamma_part, s, ev0, ev1);
+    else spmv_dispatch_lpl<VT, AT, 1>(tl, pro, epi, partial, gamma_part, s, ev0, ev1);
+}
+
+void launch_spmv_tiled(const TiledLines& tl, const Prologue& pro, const Epilogue& epi, bool acc64, void* partial,
+                       double* gamma_part, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    if (tl.n_major == 0) return;
+    if (tl.st == ST_F64) spmv_dispatch_vec<double, double>(tl, pro, epi, partial, gamma_part, s, ev0, ev1);
+    else if (tl.st == ST_F32 && acc64) spmv_dispatch_vec<float, double>(tl, pro, epi, partial, gamma_part, s, ev0, ev1);
+    else if (tl.st == ST_F32) spmv_dispatch_vec<float, float>(tl, pro, epi, partial, gamma_part, s, ev0, ev1);
+    else throw std::runtime_error("unsupported sparse storage type");
+}
+
+// ------------------------------------------------------------------ input validation
+// flag bits: 1 = pointers not starting at 0 / decreasing / not ending at nnz,
+//            2 = an index outside [0, n_secondary).
+__global__ void k_validate_compressed(const std::int32_t* __restrict__ idx, const i64* __restrict__ ptr,
+                                      i64 n_primary, i64 n_secondary, i64 nnz, int* flag) {
+    const i64 tid = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const i64 nthreads = static_cast<i64>(gridDim.x) * blockDim.x;
+    if (tid == 0 && (ptr[0] != 0 || ptr[n_primary] != nnz)) atomicOr(flag, 1);
+    for (i64 p = tid; p < n_primary; p += nthreads)
+        if (ptr[p + 1] < ptr[p]) atomicOr(flag, 1);
+    for (i64 e = tid; e < nnz; e += nthreads) {
+        const i64 v = idx[e];
+        if (v < 0 || v >= n_secondary) atomicOr(flag, 2);
+    }
+}
+
+void launch_validate_compressed(const std::int32_t* idx, const i64* ptr, i64 n_primary, i64 n_secondary, i64 nnz,
+                                int* flag, hipStream_t s) {
+    const i64 grid = std::max<i64>(1, std::min<i64>((std::max(nnz, n_primary) + 255) / 256, 16384));
+    hipLaunchKernelGGL(k_validate_compressed, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, idx, ptr,
+                       n_primary, n_secondary, nnz, flag);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ realize / column statistics
+template <typename VT>
+__global__ void k_sparse_realize(const i64* __restrict__ ptr, const std::uint16_t* __restrict__ idx,
+                                 const VT* __restrict__ val, i64 n_major, int n_tiles, int tile_len, int major_is_col,
+                                 i64 nr, double* dense) {
+    const int lane = threadIdx.x & 63;
+    const i64 wave = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const i64 nwaves = (static_cast<i64>(gridDim.x) * blockDim.x) >> 6;
+    const i64 nseg = static_cast<i64>(n_tiles) * n_major;
+    for (i64 seg = wave; seg < nseg; seg += nwaves) {
+        const i64 t = seg / n_major, m = seg - t * n_major;
+        for (i64 e = ptr[seg] + lane; e < ptr[seg + 1]; e += 64) {
+            const double v = to_double_any(val[e]);
+            if (v == 0) continue;  // padding slots (stored zeros realize to zero anyway)
+            const i64 minor = t * tile_len + idx[e];
+            const i64 r = major_is_col ? minor : m;
+            const i64 c = major_is_col ? m : minor;
+            dense[r + c * nr] = v;
+        }
+    }
+}
+
+void launch_sparse_realize(const TiledLines& t, bool major_is_col, i64 nr, double* dense, hipStream_t s) {
+    const i64 nseg = static_cast<i64>(t.n_tiles) * t.n_major;
+    const i64 grid = std::max<i64>(1, std::min<i64>((nseg * 64 + 255) / 256, 65536));
+    const dim3 g(static_cast<unsigned>(grid)), b(256);
+    const int mc = major_is_col ? 1 : 0;
+    if (t.st == ST_F64)
+        hipLaunchKernelGGL(k_sparse_realize<double>, g, b, 0, s, t.ptr, t.idx, static_cast<const double*>(t.val),
+                           t.n_major, t.n_tiles, t.tile_len, mc, nr, dense);
+    else if (t.st == ST_F32)
+        hipLaunchKernelGGL(k_sparse_realize<float>, g, b, 0, s, t.ptr, t.idx, static_cast<const float*>(t.val),
+                           t.n_major, t.n_tiles, t.tile_len, mc, nr, dense);
+    else
+        hipLaunchKernelGGL(k_sparse_realize<__hip_bfloat16>, g, b, 0, s, t.ptr, t.idx,
+                           static_cast<const __hip_bfloat16*>(t.val), t.n_major, t.n_tiles, t.tile_len, mc, nr,
+                           dense);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+// Column mean and standard deviation (pca.hpp:85-109) from the column-major
+// lines: one wave per column, two passes over its slots.  Structural zeros and
+// zero padding both enter as zeros: var = sum_slots (x-m)^2 + (nr - slots) m^2.
+template <typename VT>
+__global__ void k_sparse_colstats(const i64* __restrict__ ptr, const VT* __restrict__ val, i64 n_major, int n_tiles,
+                                  i64 nr, double* center_out, double* scale_out) {
+    const int lane = threadIdx.x & 63;
+    const i64 wave = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const i64 nwaves = (static_cast<i64>(gridDim.x) * blockDim.x) >> 6;
+    for (i64 c = wave; c < n_major; c += nwaves) {
+        double sum = 0;
+        i64 slots = 0;
+        for (int t = 0; t < n_tiles; ++t) {
+            const i64 seg = static_cast<i64>(t) * n_major + c;
+            const i64 e0 = ptr[seg], e1 = ptr[seg + 1];
+            slots += e1 - e0;
+            for (i64 e = e0 + lane; e < e1; e += 64) sum += to_double_any(val[e]);
+        }
+        sum = wave_sum(sum);
+        const double mean = center_out ? sum / nr : 0.0;
+        if (center_out && lane == 0) center_out[c] = mean;
+        if (scale_out) {
+            double ss = 0;
+            for (int t = 0; t < n_tiles; ++t) {
+                const i64 seg = static_cast<i64>(t) * n_major + c;
+                for (i64 e = ptr[seg] + lane; e < ptr[seg + 1]; e += 64) {
+                    const double d = to_double_any(val[e]) - mean;
+                    ss += d * d;
+                }
+            }
+            ss = wave_sum(ss);
+            if (lane == 0) {
+                const double var = ss + static_cast<double>(nr - slots) * mean * mean;
+                scale_out[c] = var != 0 ? sqrt(var / static_cast<double>(nr - 1)) : 1.0;
+            }
+        }
+    }
+}
+
+void launch_sparse_colstats(const TiledLines& t, i64 nr, double* center_out, double* scale_out, hipStream_t s) {
+    const i64 grid = std::max<i64>(1, std::min<i64>((t.n_major * 64 + 255) / 256, 65536));
+    const dim3 g(static_cast<unsigned>(grid)), b(256);
+    if (t.st == ST_F64)
+        hipLaunchKernelGGL(k_sparse_colstats<double>, g, b, 0, s, t.ptr, static_cast<const double*>(t.val),
+                           t.n_major, t.n_tiles, nr, center_out, scale_out);
+    else if (t.st == ST_F32)
+        hipLaunchKernelGGL(k_sparse_colstats<float>, g, b, 0, s, t.ptr, static_cast<const float*>(t.val), t.n_major,
+                           t.n_tiles, nr, center_out, scale_out);
+    else
+        hipLaunchKernelGGL(k_sparse_colstats<__hip_bfloat16>, g, b, 0, s, t.ptr,
+                           static_cast<const __hip_bfloat16*>(t.val), t.n_major, t.n_tiles, nr, center_out,
+                           scale_out);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+}  // namespace irlba_hip

@@ -1,0 +1,162 @@
+/*
+ * irlba_hip.h — C ABI of the MI355X (gfx950) IRLBA engine.
+ *
+ * Plain C types only (pointers, sizes, int status).  Every call returns
+ * IRLBA_HIP_OK (0) on success or IRLBA_HIP_ERROR (1), after which
+ * irlba_hip_last_error() holds the message (thread-local).  Messages of the
+ * errors the reference raises keep the reference's wording
+ * (include/irlba/compute.hpp:156,159,174,184; lanczos.hpp:83; pca.hpp:72,80).
+ *
+ * Arrays are column-major.  `device_pointers` = 0 means host memory (copied
+ * in/out by the call); 1 means device memory on the context's GPU.
+ * One context = one device + one stream; calls on a context are stream
+ * ordered and a context must not be used from two threads at once (the
+ * single-caller rule of a reference Workspace, include/irlba/Matrix/scaled.hpp:49).
+ * A matrix handle must not outlive its context; decorator handles share the
+ * storage of the handle they wrap (include/irlba/Matrix/centered.hpp:139-159).
+ */
+#ifndef IRLBA_HIP_H
+#define IRLBA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IRLBA_HIP_OK 0
+#define IRLBA_HIP_ERROR 1
+
+/* Element types of matrix values (input arrays and device storage). */
+#define IRLBA_HIP_F64 0
+#define IRLBA_HIP_F32 1
+#define IRLBA_HIP_BF16 2
+
+typedef struct irlba_hip_context irlba_hip_context;
+typedef struct irlba_hip_matrix irlba_hip_matrix;
+
+/* irlba::Options (include/irlba/Options.hpp:19-83), same fields and defaults. */
+typedef struct irlba_hip_options {
+    double invariant_subspace_tolerance;   /* Options.hpp:27, default -1 (eps^0.8) */
+    double convergence_tolerance;          /* Options.hpp:34, default 1e-5 */
+    double singular_value_ratio_tolerance; /* Options.hpp:41, default -1 (= convergence_tolerance) */
+    int32_t extra_work;                    /* Options.hpp:47, default 7 */
+    int32_t max_iterations;                /* Options.hpp:53, default 1000 */
+    int32_t exact_for_small_matrix;        /* Options.hpp:59, default 1 */
+    int32_t exact_for_large_number;        /* Options.hpp:65, default 1 */
+    int32_t cap_number;                    /* Options.hpp:71, default 0 */
+    int32_t reserved;
+    uint64_t seed;                         /* Options.hpp:76, default 5489 (mt19937_64::default_seed) */
+    const double* initial;                 /* Options.hpp:82, host array of length cols, or NULL */
+    int64_t initial_length;
+} irlba_hip_options;
+
+/* Counters of the last compute()/pca() on a context (no reference counterpart). */
+typedef struct irlba_hip_stats {
+    int64_t forward_products;     /* A.v calls */
+    int64_t adjoint_products;     /* A'.u calls */
+    double forward_kernel_ms;     /* summed main-kernel time of the A.v products (HIP events) */
+    double adjoint_kernel_ms;
+    double forward_bytes;         /* algorithmic bytes of one A.v product */
+    double adjoint_bytes;
+    double solve_ms;              /* wall time of the last compute() */
+    double host_svd_ms;           /* time in the host FP64 SVD of B */
+    int32_t restarts;
+    int32_t refills;              /* invariant-subspace refills (lanczos.hpp:101-106,124-130) */
+} irlba_hip_stats;
+
+/* Options defaults of Options.hpp:27-82. */
+void irlba_hip_options_default(irlba_hip_options* opt);
+
+/* Message of the last failed call on this thread. */
+const char* irlba_hip_last_error(void);
+
+/* Device context: one GPU, one stream.  No reference counterpart (the
+ * reference's parallelism is parallelize(), include/irlba/parallel.hpp:32-40). */
+int irlba_hip_context_create(int device, irlba_hip_context** out);
+int irlba_hip_context_destroy(irlba_hip_context* ctx);
+int irlba_hip_context_synchronize(irlba_hip_context* ctx);
+int irlba_hip_get_stats(const irlba_hip_context* ctx, irlba_hip_stats* out);
+/* 1 = time every product's main kernel with HIP events (default 1). */
+int irlba_hip_context_set_profiling(irlba_hip_context* ctx, int enabled);
+/* The stream the context launches on (a hipStream_t), for callers timing it. */
+void* irlba_hip_context_stream(irlba_hip_context* ctx);
+
+/* ParallelSparseMatrix(nrow, ncol, x, i, p, column_major, num_threads)
+ * (include/irlba/Matrix/sparse.hpp:438-472).  `values` has type `input_type`
+ * (F64/F32), `indices` int32, `pointers` int64 (length primary+1).  The values
+ * are stored on the device as `storage_type` (F64/F32) and products accumulate
+ * in FP64 when `accumulate_f64` != 0, else in FP32.  The arrays are copied:
+ * the handle owns device copies of both orientations. */
+int irlba_hip_sparse_create(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, int input_type, const void* values,
+                            const int32_t* indices, const int64_t* pointers, int column_major, int storage_type,
+                            int accumulate_f64, int device_pointers, irlba_hip_matrix** out);
+
+/* SimpleMatrix over a dense matrix (include/irlba/Matrix/simple.hpp:124-178).
+ * Storage F64 (FP64 accumulation), F32 or BF16 (FP32 accumulation). */
+int irlba_hip_dense_create(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, int input_type, const void* values,
+                           int column_major, int storage_type, int device_pointers, irlba_hip_matrix** out);
+
+/* CenteredMatrix(matrix, center) (include/irlba/Matrix/centered.hpp:139-159):
+ * column centring, center has length cols.  Returns a new handle. */
+int irlba_hip_centered_create(const irlba_hip_matrix* inner, const double* center, int device_pointers,
+                              irlba_hip_matrix** out);
+
+/* ScaledMatrix(matrix, scale, column, divide) (include/irlba/Matrix/scaled.hpp:195-223). */
+int irlba_hip_scaled_create(const irlba_hip_matrix* inner, const double* scale, int column, int divide,
+                            int device_pointers, irlba_hip_matrix** out);
+
+int irlba_hip_matrix_destroy(irlba_hip_matrix* m);
+
+/* Matrix::rows() / cols() (include/irlba/Matrix/interface.hpp:161,166). */
+int irlba_hip_matrix_dims(const irlba_hip_matrix* m, int64_t* rows, int64_t* cols);
+
+/* Workspace::multiply (adjoint = 0) and AdjointWorkspace::multiply (adjoint = 1)
+ * (include/irlba/Matrix/interface.hpp:49,87): output is fully overwritten. */
+int irlba_hip_multiply(const irlba_hip_matrix* m, int adjoint, const double* right, double* output,
+                       int device_pointers);
+
+/* RealizeWorkspace::realize_copy (include/irlba/Matrix/interface.hpp:119-130):
+ * rows x cols, column-major FP64. */
+int irlba_hip_realize(const irlba_hip_matrix* m, double* buffer, int device_pointers);
+
+/* compute(matrix, number, outU, outV, outD, options) (include/irlba/compute.hpp:141-308).
+ * U is rows x number, V cols x number, D number; *number_out receives the
+ * count actually written (smaller than `number` under cap_number). */
+int irlba_hip_compute(const irlba_hip_matrix* m, int64_t number, const irlba_hip_options* opt, double* U, double* V,
+                      double* D, int64_t* number_out, int32_t* converged, int32_t* iterations, int device_pointers);
+
+/* compute() over a caller-defined Matrix (interface.hpp:141-223): the products
+ * run in the caller through `matvec` on host vectors, everything else on the GPU. */
+typedef int (*irlba_hip_matvec_fn)(void* user, int adjoint, const double* right, double* output);
+int irlba_hip_compute_callback(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, irlba_hip_matvec_fn matvec,
+                               void* user, int64_t number, const irlba_hip_options* opt, double* U, double* V,
+                               double* D, int64_t* number_out, int32_t* converged, int32_t* iterations);
+
+/* Column means / standard deviations of pca() (include/irlba/pca.hpp:66-109). */
+int irlba_hip_column_stats(const irlba_hip_matrix* m, int center, int scale, double* center_out, double* scale_out,
+                           int device_pointers);
+
+/* pca(matrix, center, scale, number, scores, rotation, variances, options)
+ * (include/irlba/pca.hpp:55-139): scores rows x number, rotation cols x number. */
+int irlba_hip_pca(const irlba_hip_matrix* m, int center, int scale, int64_t number, const irlba_hip_options* opt,
+                  double* scores, double* rotation, double* variances, int64_t* number_out, int32_t* converged,
+                  int32_t* iterations, int device_pointers);
+
+/* Synthetic CSC matrix generated on the device (no reference counterpart):
+ * positions Bernoulli(density), values log1p(NB(mu=2, phi=0.5)); bit-identical
+ * to the CPU generator for the same seed. */
+int irlba_hip_generate_sparse(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, double density, uint64_t seed,
+                              int storage_type, int accumulate_f64, irlba_hip_matrix** out);
+
+/* Raw device CSC of the same generator (values as `value_type`, int32 rows,
+ * int64 column pointers) into caller-provided device arrays; call with
+ * values == NULL to get *nnz_out first. */
+int irlba_hip_generate_csc(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, double density, uint64_t seed,
+                           int value_type, void* values, int32_t* indices, int64_t* pointers, int64_t* nnz_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IRLBA_HIP_H */

@@ -1,0 +1,345 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle and
+LAPACK on the same seeded inputs.  Tolerances:
+  * FP64 products: relative 1e-12 of the output norm (different summation order).
+  * FP32/BF16 products: relative 2e-6 / 2e-2 of the output norm.
+  * FP64 solves: singular values within 1e-8 relative of the oracle's run with the
+    same explicit start vector (and of LAPACK at the reference's tolerances).
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from compare import (align_signs, assert_equal_column_vectors, assert_equal_vectors, same_same, subspace_angle)
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+irlba = pytest.importorskip("cppirlba_amd")
+
+
+def _dense_of(vals, idx, ptr, nr, nc):
+    A = np.zeros((nr, nc))
+    for c in range(nc):
+        A[idx[ptr[c]:ptr[c + 1]], c] = vals[ptr[c]:ptr[c + 1]]
+    return A
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+# ---------------------------------------------------------------- sparse products
+@pytest.mark.parametrize("nr,nc", [(10, 10), (49, 51), (97, 88), (300, 20), (20, 300)])
+@pytest.mark.parametrize("column_major", [True, False])
+def test_sparse_products_fp64(nr, nc, column_major):
+    # tests/src/Matrix/sparse.cpp:91-148 on the device.
+    if column_major:
+        vals, idx, ptr = O.simulate_compressed_sparse(nr, nc)
+        dense = _dense_of(vals, idx, ptr, nr, nc)
+        A = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, column_major=True)
+    else:
+        vals, idx, ptr = O.simulate_compressed_sparse(nc, nr)  # CSC of the transpose = CSR of dense
+        dense = _dense_of(vals, idx, ptr, nc, nr).T
+        A = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, column_major=False)
+    assert A.shape == (nr, nc)
+    v = O.create_random_vector(nc, 17)
+    u = O.create_random_vector(nr, 18)
+    for _ in range(2):
+        y = A.multiply(v)
+        z = A.adjoint_multiply(u)
+        assert _rel(y, dense @ v) < 1e-13
+        assert _rel(z, dense.T @ u) < 1e-13
+    np.testing.assert_array_equal(A.realize(), dense)
+
+
+def test_sparse_multitile_both_orientations():
+    # 40000 columns exceed one FP64 LDS tile, so A.v runs with several tiles and partials.
+    nr, nc = 64, 40000
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.05, 7, f32=False)
+    A = O.sparse(nr, nc, vals, idx, ptr, True, 1)
+    H = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr)
+    v = O.create_random_vector(nc, 3)
+    u = O.create_random_vector(nr, 4)
+    assert _rel(H.multiply(v), A.multiply(v)) < 1e-12
+    assert _rel(H.adjoint_multiply(u), A.multiply(u, adjoint=True)) < 1e-12
+    # and the transpose shape: many row tiles on the adjoint side
+    Ht = irlba.HipSparseMatrix(nc, nr, vals, idx, ptr, column_major=False)
+    At = O.sparse(nc, nr, vals, idx, ptr, False, 1)
+    assert _rel(Ht.multiply(u), At.multiply(u)) < 1e-12
+    assert _rel(Ht.adjoint_multiply(v), At.multiply(v, adjoint=True)) < 1e-12
+
+
+def test_sparse_mixed_precision_products():
+    nr, nc = 3000, 5000
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.03, 11, f32=True)
+    A = O.sparse(nr, nc, vals, idx, ptr, True, 1)  # float values, double accumulation (reference semantics)
+    H32 = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, accumulate="f32")
+    H64 = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, accumulate="f64")
+    v = O.create_random_vector(nc, 5)
+    u = O.create_random_vector(nr, 6)
+    assert _rel(H64.multiply(v), A.multiply(v)) < 1e-13
+    assert _rel(H64.adjoint_multiply(u), A.multiply(u, adjoint=True)) < 1e-13
+    assert _rel(H32.multiply(v), A.multiply(v)) < 2e-6
+    assert _rel(H32.adjoint_multiply(u), A.multiply(u, adjoint=True)) < 2e-6
+
+
+def test_sparse_rejects_bad_input():
+    vals = np.ones(3)
+    with pytest.raises(irlba.HipError, match="indices"):
+        irlba.HipSparseMatrix(4, 2, vals, np.array([0, 5, 1]), np.array([0, 2, 3]))
+    with pytest.raises(irlba.HipError, match="pointers"):
+        irlba.HipSparseMatrix(4, 2, vals, np.array([0, 1, 1]), np.array([0, 3, 2]))
+    with pytest.raises(irlba.HipError, match="more than once"):
+        irlba.HipSparseMatrix(4, 2, vals, np.array([1, 1, 2]), np.array([0, 2, 3]))
+
+
+def test_generator_bit_identical_to_oracle():
+    nr, nc, d, seed = 700, 300, 0.04, 99
+    v, i, p = irlba.generate_csc_host(nr, nc, d, seed, "f32")
+    ov, oi, op = O.gen_nb_sparse(nr, nc, d, seed, f32=True)
+    np.testing.assert_array_equal(p, op)
+    np.testing.assert_array_equal(i, oi)
+    np.testing.assert_array_equal(v, ov)
+
+
+# ---------------------------------------------------------------- dense products
+@pytest.mark.parametrize("nr,nc", [(20, 10), (1000, 200), (33, 1001), (4099, 17)])
+@pytest.mark.parametrize("order", ["F", "C"])
+def test_dense_products(nr, nc, order):
+    rng = np.random.default_rng(nr + nc)
+    A = np.asarray(rng.standard_normal((nr, nc)), order=order)
+    v = rng.standard_normal(nc)
+    u = rng.standard_normal(nr)
+    H = irlba.HipDenseMatrix(A, storage="f64")
+    assert _rel(H.multiply(v), A @ v) < 1e-13
+    assert _rel(H.adjoint_multiply(u), A.T @ u) < 1e-13
+    H32 = irlba.HipDenseMatrix(A.astype(np.float32), storage="f32")
+    assert _rel(H32.multiply(v), A.astype(np.float32).astype(np.float64) @ v) < 2e-6
+    Hb = irlba.HipDenseMatrix(A, storage="bf16")
+    assert _rel(Hb.multiply(v), A @ v) < 2e-2
+    assert _rel(Hb.adjoint_multiply(u), A.T @ u) < 2e-2
+
+
+# ---------------------------------------------------------------- decorators
+@pytest.mark.parametrize("column", [True, False])
+@pytest.mark.parametrize("divide", [True, False])
+@pytest.mark.parametrize("kind", ["dense", "sparse"])
+def test_centered_scaled_products(column, divide, kind):
+    # tests/src/Matrix/centered.cpp, scaled.cpp on the device.
+    nr, nc = 60, 45
+    if kind == "dense":
+        A = O.create_random_matrix(nr, nc)
+        H = irlba.HipDenseMatrix(A)
+        OM = O.dense(A)
+    else:
+        vals, idx, ptr = O.simulate_compressed_sparse(nr, nc)
+        H = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr)
+        OM = O.sparse(nr, nc, vals, idx, ptr, True, 1)
+    c = O.create_random_vector(nc, 1)
+    s = np.abs(O.create_random_vector(nc if column else nr, 2)) + 0.5
+    HC = irlba.CenteredMatrix(H, c)
+    HS = irlba.ScaledMatrix(HC, s, column, divide)
+    OS = O.scaled(O.centered(OM, c), s, column, divide)
+    x = O.create_random_vector(nc, 3)
+    u = O.create_random_vector(nr, 4)
+    for _ in range(2):
+        assert _rel(HS.multiply(x), OS.multiply(x)) < 1e-13
+        assert _rel(HS.adjoint_multiply(u), OS.multiply(u, adjoint=True)) < 1e-13
+    np.testing.assert_allclose(HS.realize(), OS.realize(), rtol=1e-13, atol=1e-13)
+    # the other decorator order, and two centring layers (merged on the device)
+    HSC = irlba.CenteredMatrix(irlba.ScaledMatrix(H, s, column, divide), c)
+    OSC = O.centered(O.scaled(OM, s, column, divide), c)
+    assert _rel(HSC.multiply(x), OSC.multiply(x)) < 1e-13
+    assert _rel(HSC.adjoint_multiply(u), OSC.multiply(u, adjoint=True)) < 1e-13
+    HCC = irlba.CenteredMatrix(HC, c * 0.5)
+    OCC = O.centered(O.centered(OM, c), c * 0.5)
+    assert _rel(HCC.multiply(x), OCC.multiply(x)) < 1e-13
+
+
+# ---------------------------------------------------------------- compute
+def test_compute_compare_to_exact():
+    # tests/src/compute.cpp:15-31
+    A = O.create_random_matrix(20, 10)
+    res = irlba.compute(irlba.HipDenseMatrix(A), 5, irlba.Options(exact_for_large_number=False))
+    Ul, Sl, Vlt = np.linalg.svd(A, full_matrices=False)
+    assert_equal_vectors(res.D, Sl[:5], 1e-8)
+    assert_equal_column_vectors(res.U, Ul[:, :5], 1e-8)
+    assert_equal_column_vectors(res.V, Vlt.T[:, :5], 1e-8)
+
+
+@pytest.mark.parametrize("nr", [20, 50, 100])
+@pytest.mark.parametrize("nc", [20, 50, 100])
+@pytest.mark.parametrize("rank", [2, 5, 10])
+def test_compute_sweep_vs_lapack_and_oracle(nr, nc, rank):
+    # tests/src/compute.cpp:78-123 on the device, plus the oracle on the same start.
+    A = O.create_random_matrix(nr, nc)
+    init = O.create_random_vector(nc, 1239 + nr + nc + rank)
+    H = irlba.HipDenseMatrix(A)
+    res = irlba.compute(H, rank, irlba.Options(convergence_tolerance=1e-9, initial=init))
+    Ul, Sl, Vlt = np.linalg.svd(A, full_matrices=False)
+    assert_equal_vectors(res.D, Sl[:rank], 1e-6)
+    assert_equal_column_vectors(res.U, Ul[:, :rank], 1e-6)
+    assert_equal_column_vectors(res.V, Vlt.T[:, :rank], 1e-6)
+    oU, oV, oD, oconv, oit = O.compute(O.dense(A), rank, convergence_tolerance=1e-9, initial=init)
+    assert_equal_vectors(res.D, oD, 1e-8)
+    assert res.converged == oconv
+
+
+def test_compute_sparse_dense_parallel_agree():
+    # tests/src/compute.cpp:240-285
+    vals, idx, ptr = O.simulate_compressed_sparse(99, 64)
+    dense = _dense_of(vals, idx, ptr, 99, 64)
+    opt = irlba.Options(extra_work=7)
+    r1 = irlba.compute(irlba.HipDenseMatrix(dense), 8, opt)
+    r2 = irlba.compute(irlba.HipSparseMatrix(99, 64, vals, idx, ptr), 8, opt)
+    assert_equal_vectors(r1.D, r2.D)
+    assert_equal_column_vectors(r1.U, r2.U)
+    assert_equal_column_vectors(r1.V, r2.V)
+    r3 = irlba.compute(irlba.HipSparseMatrix(99, 64, vals, idx, ptr), 13, irlba.Options(extra_work=20))
+    Ul, Sl, Vlt = np.linalg.svd(dense, full_matrices=False)
+    assert_equal_vectors(r3.D, Sl[:13], 1e-5)
+    assert_equal_column_vectors(r3.U, Ul[:, :13], 1e-5)
+    assert_equal_column_vectors(r3.V, Vlt.T[:, :13], 1e-5)
+
+
+def test_compute_small_large_exact_and_errors():
+    # tests/src/compute.cpp:157-238
+    small = O.create_random_matrix(10, 3)
+    res = irlba.compute(irlba.HipDenseMatrix(small), 2)
+    assert res.converged and res.iterations == 0
+    np.testing.assert_allclose(res.D, np.linalg.svd(small, compute_uv=False)[:2], rtol=1e-12)
+    mat = O.create_random_matrix(20, 50)
+    H = irlba.HipDenseMatrix(mat)
+    Sl = np.linalg.svd(mat, compute_uv=False)
+    np.testing.assert_allclose(irlba.compute(H, 15).D, Sl[:15], rtol=1e-12)
+    r = irlba.compute(H, 50, irlba.Options(cap_number=True))
+    assert r.D.shape == (20,) and r.U.shape == (20, 20) and r.V.shape == (50, 20)
+    A = irlba.HipDenseMatrix(O.create_random_matrix(20, 10))
+    with pytest.raises(irlba.HipError, match="cannot be greater than"):
+        irlba.compute(A, 100)
+    with pytest.raises(irlba.HipError, match="must be less than"):
+        irlba.compute(A, 10, irlba.Options(exact_for_large_number=False))
+    with pytest.raises(irlba.HipError, match="must be positive"):
+        irlba.compute(A, 0, irlba.Options(extra_work=0))
+    with pytest.raises(irlba.HipError) as e:
+        irlba.compute(A, 2, irlba.Options(initial=np.zeros(1)))
+    assert str(e.value).startswith("initialization")
+
+
+def test_null_space_start_throws():
+    A = np.zeros((30, 20))
+    A[:, 0] = 1.0
+    init = np.zeros(20)
+    init[1] = 1.0
+    with pytest.raises(irlba.HipError, match="starting vector near the null space"):
+        irlba.compute(irlba.HipDenseMatrix(A), 2, irlba.Options(initial=init))
+
+
+@pytest.mark.parametrize("rank", [2, 5, 10])
+def test_invariant_subspace_refills(rank):
+    # tests/src/invariant.cpp:31-80: exact low rank drives the refill path.
+    nr, nc = 100, 50
+    base = O.create_random_matrix(nr, rank, 42)
+    A = np.empty((nr, nc))
+    for i in range(nc):
+        A[:, i] = base[:, i % rank]
+    H = irlba.HipDenseMatrix(A)
+    res = irlba.compute(H, rank + 3)
+    assert np.all(res.D[:rank] > 0.01)
+    assert np.all(res.D[rank:] < 1e-8)
+    assert same_same(np.sum(res.D ** 2), np.sum(A ** 2), 1e-8)
+    assert H.ctx.stats()["refills"] > 0
+    res = irlba.compute(H, rank + 2)
+    np.testing.assert_allclose(res.U @ np.diag(res.D) @ res.V.T, A, atol=1e-8)
+
+
+def test_cfg1_planted_dense_vs_oracle():
+    # Dense 1000x200 FP64 = U diag(10*0.8^i) V' + 0.01 N(0,1); top-5, extra_work=20,
+    # max_iterations=50, tol=1e-5, explicit start vector (seed 1234).
+    rng = np.random.default_rng(20240601)
+    U0, _ = np.linalg.qr(rng.standard_normal((1000, 200)))
+    V0, _ = np.linalg.qr(rng.standard_normal((200, 200)))
+    s = 10 * 0.8 ** np.arange(200)
+    A = (U0 * s) @ V0.T + 0.01 * rng.standard_normal((1000, 200))
+    init = np.random.default_rng(1234).standard_normal(200)
+    opt = irlba.Options(extra_work=20, max_iterations=50, convergence_tolerance=1e-5, initial=init)
+    res = irlba.compute_simple(A, 5, opt)
+    oU, oV, oD, oconv, oit = O.compute(O.dense(A), 5, extra_work=20, max_iterations=50, convergence_tolerance=1e-5,
+                                       initial=init)
+    assert res.converged and oconv
+    assert_equal_vectors(res.D, oD, 1e-8)
+    assert subspace_angle(res.U, oU) < 1e-6 and subspace_angle(res.V, oV) < 1e-6
+    Sl = np.linalg.svd(A, compute_uv=False)
+    assert_equal_vectors(res.D, Sl[:5], 1e-8)
+
+
+# ---------------------------------------------------------------- pca
+@pytest.mark.parametrize("center,scale", [(True, True), (True, False), (False, True), (False, False)])
+def test_pca_dense_vs_oracle(center, scale):
+    # tests/src/pca.cpp:66-141
+    A = O.create_random_matrix(100, 50)
+    init = O.create_random_vector(50, 3)
+    res = irlba.pca(A, center, scale, 5, irlba.Options(initial=init, convergence_tolerance=1e-9))
+    sc, rot, var, conv, it = O.pca(O.dense(A), center, scale, 5, initial=init, convergence_tolerance=1e-9)
+    assert_equal_vectors(res.variances, var, 1e-8)
+    assert_equal_column_vectors(res.rotation, rot, 1e-6)
+    assert_equal_column_vectors(res.scores, sc, 1e-6, centered=center)
+    if center:
+        assert np.all(np.abs(res.scores.sum(axis=0)) < 1e-8)
+        for i in range(5):
+            assert same_same((res.scores[:, i] ** 2).sum() / 99, res.variances[i], 1e-8)
+
+
+def test_pca_sparse_vs_oracle_and_stats():
+    # tests/src/pca.cpp:218-240 with the NB generator.
+    nr, nc = 400, 600
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.05, 21, f32=False)
+    H = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr)
+    OM = O.sparse(nr, nc, vals, idx, ptr, True, 1)
+    c, s = irlba.column_stats(H)
+    oc, os_ = O.column_stats(OM)
+    np.testing.assert_allclose(c, oc, rtol=1e-13)
+    np.testing.assert_allclose(s, os_, rtol=1e-12)
+    init = O.create_random_vector(nc, 8)
+    res = irlba.pca(H, True, True, 10, irlba.Options(extra_work=20, initial=init, convergence_tolerance=1e-8))
+    sc, rot, var, conv, it = O.pca(OM, True, True, 10, extra_work=20, initial=init, convergence_tolerance=1e-8)
+    assert_equal_vectors(res.variances, var, 1e-8)
+    assert subspace_angle(res.rotation, rot) < 1e-5
+    assert np.all(np.abs(res.scores.sum(axis=0)) < 1e-8)
+
+
+def test_pca_failures_and_zero_variance():
+    with pytest.raises(irlba.HipError, match="cannot scale"):
+        irlba.pca(O.create_random_matrix(1, 10), True, True, 1)
+    out = irlba.pca(np.zeros((10, 2)), True, True, 1)
+    assert out.converged and np.isfinite(out.scores[0, 0]) and np.isfinite(out.rotation[0, 0])
+
+
+# ---------------------------------------------------------------- caller-defined matrices
+class NumpyMatrix(irlba.Matrix):
+    def __init__(self, A):
+        self.A = A
+
+    def rows(self):
+        return self.A.shape[0]
+
+    def cols(self):
+        return self.A.shape[1]
+
+    def multiply(self, x):
+        return self.A @ x
+
+    def adjoint_multiply(self, x):
+        return self.A.T @ x
+
+
+def test_callback_matrix_matches_device_matrix():
+    A = O.create_random_matrix(80, 60)
+    init = O.create_random_vector(60, 9)
+    opt = irlba.Options(initial=init)
+    r1 = irlba.compute(NumpyMatrix(A), 6, opt)
+    r2 = irlba.compute(irlba.HipDenseMatrix(A), 6, opt)
+    assert_equal_vectors(r1.D, r2.D, 1e-10)
+    assert_equal_column_vectors(r1.U, r2.U, 1e-6)
