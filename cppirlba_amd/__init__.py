@@ -429,3 +429,42 @@ def generate_csc_host(nrow, ncol, density, seed, value_type="f32", ctx=None):
                                        ctypes.byref(nnz)))
     ctx.synchronize()
     return vals.cpu().numpy(), idx.cpu().numpy(), ptr.cpu().numpy()
+
+
+def _colmajor(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def run_lanczos_bidiagonalization(matrix, W, V, B, seed, start, options=None):
+    """One sweep of lanczos.hpp:54-143 on the device; returns (W, V, B, F)."""
+    options = options or Options()
+    o, keep = options._to_c()
+    W = _colmajor(W).copy(order="F")
+    V = _colmajor(V).copy(order="F")
+    B = _colmajor(B).copy(order="F")
+    F = np.zeros(matrix.cols())
+    check(lib().irlba_hip_lanczos(matrix.handle, W.shape[1], W.ctypes.data, V.ctypes.data, B.ctypes.data, seed, start,
+                                  ctypes.byref(o), F.ctypes.data))
+    return W, V, B, F
+
+
+def orthogonalize_vector(M, vec, ncols, ctx=None):
+    """lanczos.hpp:17-21 on the device: vec - M[:, :ncols] M[:, :ncols]' vec; returns (vec, norm)."""
+    ctx = ctx or default_context()
+    M = _colmajor(M)
+    v = np.ascontiguousarray(vec, dtype=np.float64).copy()
+    nrm = ctypes.c_double()
+    check(lib().irlba_hip_orthogonalize(ctx.handle, M.ctypes.data, M.shape[0], M.shape[1], v.ctypes.data, ncols,
+                                        ctypes.byref(nrm)))
+    return v, nrm.value
+
+
+def rotate(M, Y, ctx=None):
+    """The restart rotation M Y (compute.hpp:268,279) on the matrix cores."""
+    ctx = ctx or default_context()
+    M = _colmajor(M)
+    Y = _colmajor(Y)
+    out = np.zeros((M.shape[0], Y.shape[1]), order="F")
+    check(lib().irlba_hip_rotate(ctx.handle, M.ctypes.data, M.shape[0], M.shape[1], Y.ctypes.data, Y.shape[1],
+                                 out.ctypes.data))
+    return out

@@ -93,6 +93,14 @@ def lib():
     """Load libirlba_hip.so (raises if it was never built)."""
     global _lib
     if _lib is None:
+        # torch ships its own HIP runtime under the same soname (libamdhip64.so.7,
+        # libhsa-runtime64.so.1).  Loading it first makes this library bind to that
+        # copy, so a process never holds two HIP runtimes (the second one to open
+        # the GPU finds no device).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                               "(python cppirlba_amd/build.py)")
@@ -108,3 +116,10 @@ def lib():
 def check(rc):
     if rc != 0:
         raise HipError(lib().irlba_hip_last_error().decode())
+
+
+SIGNATURES.update({
+    "irlba_hip_lanczos": (_i, [_VP, _i64, _VP, _VP, _VP, ctypes.c_uint64, _i64, ctypes.POINTER(Options), _VP]),
+    "irlba_hip_orthogonalize": (_i, [_VP, _VP, _i64, _i64, _VP, _i64, _VP]),
+    "irlba_hip_rotate": (_i, [_VP, _VP, _i64, _i64, _VP, _i64, _VP]),
+})

@@ -1278,6 +1278,87 @@ int irlba_hip_compute_callback(irlba_hip_context* ctx, int64_t nrow, int64_t nco
     });
 }
 
+int irlba_hip_lanczos(const irlba_hip_matrix* m, int64_t work, double* W, double* V, double* B, uint64_t seed,
+                      int64_t start, const irlba_hip_options* opt, double* F) {
+    return guarded([&] {
+        require(m && W && V && B && F, "null argument");
+        require(work > 0 && start >= 0 && start < work, "start must lie within the working subspace");
+        irlba_hip_context* ctx = m->ctx;
+        DeviceGuard g(ctx->device);
+        HandleOp op;
+        op.ctx = ctx;
+        op.nr = m->base->nr;
+        op.nc = m->base->nc;
+        op.m = m;
+        Opts o = to_opts(opt);
+        o.seed = seed;
+        Engine eg(ctx, op, o, work);
+        const i64 nr = op.nr, nc = op.nc;
+        IRLBA_HIP_CHECK(hipMemcpy(eg.W.get(), W, sizeof(double) * nr * work, hipMemcpyHostToDevice));
+        IRLBA_HIP_CHECK(hipMemcpy(eg.V.get(), V, sizeof(double) * nc * work, hipMemcpyHostToDevice));
+        HostMat hb(work, work);
+        std::copy(B, B + work * work, hb.a.begin());
+        eg.sweep(start, hb);
+        std::copy(hb.a.begin(), hb.a.end(), B);
+        IRLBA_HIP_CHECK(hipMemcpy(W, eg.W.get(), sizeof(double) * nr * work, hipMemcpyDeviceToHost));
+        IRLBA_HIP_CHECK(hipMemcpy(V, eg.V.get(), sizeof(double) * nc * work, hipMemcpyDeviceToHost));
+        IRLBA_HIP_CHECK(hipMemcpy(F, eg.F.get(), sizeof(double) * nc, hipMemcpyDeviceToHost));
+    });
+}
+
+int irlba_hip_orthogonalize(irlba_hip_context* ctx, const double* M, int64_t rows, int64_t total_cols, double* vec,
+                            int64_t ncols, double* norm_out) {
+    return guarded([&] {
+        require(ctx && M && vec, "null argument");
+        require(ncols >= 0 && ncols <= total_cols, "ncols must not exceed the columns of the basis");
+        DeviceGuard g(ctx->device);
+        hipStream_t s = ctx->stream;
+        auto dM = dalloc<double>(rows * std::max<i64>(total_cols, 1));
+        auto df = dalloc<double>(rows);
+        const int G = cgs_blocks(rows);
+        auto part = dalloc<double>(static_cast<i64>(G) * std::max<i64>(ncols, 1));
+        auto t = dalloc<double>(std::max<i64>(ncols, 1));
+        auto np = dalloc<double>(G);
+        auto nrm = dalloc<double>(1);
+        IRLBA_HIP_CHECK(hipMemcpy(dM.get(), M, sizeof(double) * rows * total_cols, hipMemcpyHostToDevice));
+        IRLBA_HIP_CHECK(hipMemcpy(df.get(), vec, sizeof(double) * rows, hipMemcpyHostToDevice));
+        if (ncols > 0) {
+            launch_cgs_dots(dM.get(), rows, static_cast<int>(ncols), df.get(), rows, part.get(), s);
+            launch_cgs_reduce(part.get(), G, static_cast<int>(ncols), t.get(), s);
+            launch_cgs_update(dM.get(), rows, static_cast<int>(ncols), t.get(), df.get(), rows, np.get(), s);
+        } else {
+            launch_sumsq(df.get(), rows, np.get(), s);
+        }
+        launch_normalize(df.get(), rows, np.get(), G, nrm.get(), nullptr, s);
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+        IRLBA_HIP_CHECK(hipMemcpy(vec, df.get(), sizeof(double) * rows, hipMemcpyDeviceToHost));
+        if (norm_out) IRLBA_HIP_CHECK(hipMemcpy(norm_out, nrm.get(), sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+int irlba_hip_rotate(irlba_hip_context* ctx, const double* M, int64_t rows, int64_t work, const double* Y, int64_t k,
+                     double* out) {
+    return guarded([&] {
+        require(ctx && M && Y && out, "null argument");
+        require(k >= 0 && k <= work, "k must not exceed work");
+        DeviceGuard g(ctx->device);
+        hipStream_t s = ctx->stream;
+        const int ldy = static_cast<int>((work + 15) / 16 * 16);
+        std::vector<double> yt(static_cast<size_t>(work * ldy), 0.0);
+        for (i64 r = 0; r < work; ++r)
+            for (i64 c = 0; c < k; ++c) yt[r * ldy + c] = Y[r + c * work];
+        auto dM = dalloc<double>(rows * work);
+        auto dY = dalloc<double>(work * ldy);
+        auto dO = dalloc<double>(rows * std::max<i64>(k, 1));
+        IRLBA_HIP_CHECK(hipMemcpy(dM.get(), M, sizeof(double) * rows * work, hipMemcpyHostToDevice));
+        IRLBA_HIP_CHECK(hipMemcpy(dY.get(), yt.data(), sizeof(double) * yt.size(), hipMemcpyHostToDevice));
+        launch_rotate(dM.get(), rows, static_cast<int>(work), dY.get(), ldy, static_cast<int>(k), dO.get(), rows, rows,
+                      s);
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+        IRLBA_HIP_CHECK(hipMemcpy(out, dO.get(), sizeof(double) * rows * k, hipMemcpyDeviceToHost));
+    });
+}
+
 int irlba_hip_column_stats(const irlba_hip_matrix* m, int center, int scale, double* center_out, double* scale_out,
                            int device_pointers) {
     return guarded([&] {

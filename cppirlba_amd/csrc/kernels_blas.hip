@@ -226,13 +226,13 @@ void launch_scale_columns(double* M, i64 ld, i64 L, int k, const double* d, hipS
 // kRotTiles*16 columns with v_mfma_f64_16x16x4f64:
 //   A operand: lane l holds M[row0 + l%16, kk + l/16]
 //   B operand: lane l holds Y[kk + l/16, col0 + l%16]
-//   D:         lane l, element i is out[row0 + 4*(l/16) + i, col0 + l%16]
+//   D:         lane l, element i is out[row0 + l/16 + 4*i, col0 + l%16]  (checked on gfx950)
 // All reads of a strip precede its writes, so out == M is safe when one pass
 // covers every output column (k <= kRotTiles*16).
 typedef double double4_t __attribute__((ext_vector_type(4)));
 constexpr int kRotTiles = 8;
 
-__global__ __launch_bounds__(256) void k_rotate(const double* __restrict__ M, i64 ld, int work,
+__global__ __launch_bounds__(256) void k_rotate(const double* M, i64 ld, int work,
                                                 const double* __restrict__ Yt, int ldy, int k, int ct_begin,
                                                 double* out, i64 ld_out, i64 L) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(256) void k_rotate(const double* __restrict__ M, i6
             if (col < k) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const i64 row = row0 + 4 * lk + i;
+                    const i64 row = row0 + lk + 4 * i;
                     if (row < L) out[row + static_cast<i64>(col) * ld_out] = acc[c][i];
                 }
             }

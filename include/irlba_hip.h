@@ -133,6 +133,25 @@ int irlba_hip_compute_callback(irlba_hip_context* ctx, int64_t nrow, int64_t nco
                                void* user, int64_t number, const irlba_hip_options* opt, double* U, double* V,
                                double* D, int64_t* number_out, int32_t* converged, int32_t* iterations);
 
+/* run_lanczos_bidiagonalization(work, W, V, B, eng, start, options)
+ * (include/irlba/lanczos.hpp:54-143): one sweep from column `start` on the
+ * caller's W (rows x work), V (cols x work) and B (work x work), all host,
+ * column-major, updated in place; F (cols) receives the residual vector.  The
+ * random engine is std::mt19937_64(seed). */
+int irlba_hip_lanczos(const irlba_hip_matrix* m, int64_t work, double* W, double* V, double* B, uint64_t seed,
+                      int64_t start, const irlba_hip_options* opt, double* F);
+
+/* orthogonalize_vector(mat, vec, ncols, tmp) (include/irlba/lanczos.hpp:17-21):
+ * vec -= M[:, :ncols] (M[:, :ncols]' vec) for a host rows x total_cols matrix M;
+ * *norm_out (if not NULL) receives the norm of the result. */
+int irlba_hip_orthogonalize(irlba_hip_context* ctx, const double* M, int64_t rows, int64_t total_cols, double* vec,
+                            int64_t ncols, double* norm_out);
+
+/* The restart rotation out = M[:, :work] Y[:, :k] (include/irlba/compute.hpp:268,279)
+ * on the matrix cores; host arrays, M rows x work, Y work x k, out rows x k. */
+int irlba_hip_rotate(irlba_hip_context* ctx, const double* M, int64_t rows, int64_t work, const double* Y, int64_t k,
+                     double* out);
+
 /* Column means / standard deviations of pca() (include/irlba/pca.hpp:66-109). */
 int irlba_hip_column_stats(const irlba_hip_matrix* m, int center, int scale, double* center_out, double* scale_out,
                            int device_pointers);
