@@ -70,8 +70,48 @@ def cpu_baseline(nr, ncol_full, density, seed, nnz_full, n_fwd, n_adj, sample_co
     }
 
 
+def spmv_mode(args):
+    """Products only (for rocprofv3 counter runs): `reps` A.v and A'.u products of
+    the mixed-precision matrix, then a device copy of a known size whose
+    FETCH_SIZE/WRITE_SIZE calibrate the counters."""
+    import ctypes
+
+    import torch
+
+    import cppirlba_amd as irlba
+    from cppirlba_amd import _lib
+
+    ctx = irlba.default_context()
+    storage = args.storage
+    mat = irlba.generate_sparse(args.rows, args.cols, args.density, args.seed, storage=storage,
+                                accumulate="f64" if storage == "f64" else "f32")
+    ctx.synchronize()
+    L = _lib.lib()
+    x = torch.randn(args.cols, dtype=torch.float64, device="cuda")
+    y = torch.randn(args.rows, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(args.reps):
+        _lib.check(L.irlba_hip_multiply(mat.handle, 0, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), 1))
+        _lib.check(L.irlba_hip_multiply(mat.handle, 1, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()), 1))
+    st = ctx.stats()
+    n = 1 << 27  # 1 GiB of FP64
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": "spmv", "storage": storage, "reps": args.reps,
+                      "forward_ms": st["forward_kernel_ms"] / max(1, st["forward_products"]),
+                      "adjoint_ms": st["adjoint_kernel_ms"] / max(1, st["adjoint_products"]),
+                      "forward_bytes": st["forward_bytes"], "adjoint_bytes": st["adjoint_bytes"],
+                      "copy_bytes": n * 8}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--mode", choices=["solve", "spmv"], default="solve")
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--storage", default="f32")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
@@ -90,6 +130,9 @@ def main():
         print("bench.py runs on one GPU only: the IRLBA solve is not sharded across GPUs in this build "
               "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "spmv":
+        spmv_mode(args)
+        return
 
     import numpy as np
 

@@ -98,11 +98,33 @@ struct Epilogue {                  // out[m] = oscale[m]*acc - sum_k tvec_k[m]*g
     const double* axpy_coef = nullptr;  // device scalar
 };
 
-__device__ __forceinline__ double apply_epilogue(const Epilogue& e, i64 m, double acc, const double* gamma) {
-    double o = e.oscale ? acc * e.oscale[m] : acc;
-    for (int k = 0; k < e.n_terms; ++k) o -= (e.tvec[k] ? e.tvec[k][m] : 1.0) * gamma[k];
-    if (e.axpy_src) o -= (*e.axpy_coef) * e.axpy_src[m];
+// The per-output operands of the epilogue, loaded before the output's
+// reduction so their latency overlaps the stream of its nonzeros.
+struct EpiOperands {
+    double scale = 1.0, t0 = 1.0, t1 = 1.0, src = 0.0;
+};
+
+__device__ __forceinline__ EpiOperands load_epilogue(const Epilogue& e, i64 m) {
+    EpiOperands o;
+    if (e.oscale) o.scale = e.oscale[m];
+    if (e.n_terms > 0 && e.tvec[0]) o.t0 = e.tvec[0][m];
+    if (e.n_terms > 1 && e.tvec[1]) o.t1 = e.tvec[1][m];
+    if (e.axpy_src) o.src = e.axpy_src[m];
     return o;
+}
+
+// `coef` is *e.axpy_coef, read once per kernel (0 when there is no axpy).
+__device__ __forceinline__ double finish_epilogue(const Epilogue& e, const EpiOperands& p, double acc,
+                                                  const double* gamma, double coef) {
+    double o = acc * p.scale;
+    if (e.n_terms > 0) o -= p.t0 * gamma[0];
+    if (e.n_terms > 1) o -= p.t1 * gamma[1];
+    if (e.axpy_src) o -= coef * p.src;
+    return o;
+}
+
+__device__ __forceinline__ double apply_epilogue(const Epilogue& e, i64 m, double acc, const double* gamma) {
+    return finish_epilogue(e, load_epilogue(e, m), acc, gamma, e.axpy_src ? *e.axpy_coef : 0.0);
 }
 
 // ------------------------------------------------------------------ tiled compressed lines
@@ -116,12 +138,12 @@ struct TiledLines {
     int tile_len = 0, n_tiles = 0;
     i64 nnz = 0, slots = 0;     // true entries, stored slots (with padding)
     int vec = 1, lpl = 64;      // vector width per lane, lanes per line
-    int G = 1;                  // nnz-balanced chunks per tile
+    int B = 1;                  // blocks of one product (one resident wave)
     StorageType st = ST_F64;
     i64* ptr = nullptr;         // device, n_tiles*n_major + 1
     std::uint16_t* idx = nullptr;
     void* val = nullptr;
-    int* chunk = nullptr;       // device, n_tiles*(G+1) line boundaries
+    i64* chunk = nullptr;       // device, B+1 segment boundaries (nnz quantiles)
 };
 
 // ------------------------------------------------------------------ launchers (kernels_*.hip)
@@ -137,7 +159,7 @@ void launch_scatter_entries(const std::int32_t* idx, const void* val_in, Storage
 void launch_sort_segments(const i64* ptr, const unsigned long long* counts, i64 n_seg, int tile_len,
                           const std::uint16_t* in_idx, const void* in_val, std::uint16_t* out_idx, void* out_val,
                           StorageType st, int* dup_flag, hipStream_t s);
-void launch_chunk_bounds(const i64* ptr, i64 n_major, int n_tiles, int G, int* chunk, hipStream_t s);
+void launch_chunk_bounds(const i64* ptr, i64 nseg, int B, i64* chunk, hipStream_t s);
 // Sparse products.
 void launch_spmv_tiled(const TiledLines& t, const Prologue& pro, const Epilogue& epi, bool acc64, void* partial,
                        double* gamma_part, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
@@ -164,7 +186,8 @@ i64 dense_partial_elems(const DenseDesc& d);
 // Lanczos vector kernels (FP64).
 int cgs_blocks(i64 L);
 void launch_cgs_dots(const double* M, i64 ld, int n, const double* f, i64 L, double* partials, hipStream_t s);
-void launch_cgs_reduce(const double* partials, int G, int n, double* t, hipStream_t s);
+int cgs_dot_blocks(i64 L);
+void launch_cgs_reduce(const double* partials, i64 L, int n, double* t, hipStream_t s);
 void launch_cgs_update(const double* M, i64 ld, int n, const double* t, double* f, i64 L, double* norm_part,
                        hipStream_t s);
 void launch_sumsq(const double* f, i64 L, double* norm_part, hipStream_t s);

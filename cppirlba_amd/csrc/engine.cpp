@@ -85,6 +85,7 @@ struct irlba_hip_context {
     hipStream_t stream = nullptr;
     int cu_count = 256;
     size_t lds_per_block = 65536;
+    size_t lds_per_cu = 65536;
     bool profiling = true;
     irlba_hip_stats stats{};
     struct Pending {
@@ -255,7 +256,7 @@ void apply_handle(const irlba_hip_matrix& m, bool adjoint, const double* x, doub
 // ------------------------------------------------------------------ sparse construction
 TiledLines build_lines(irlba_hip_context* ctx, const std::int32_t* d_idx, const void* d_val, StorageType in_t,
                        const i64* d_ptr, i64 n_primary, bool transpose, i64 n_major_out, i64 n_minor, int tile_len,
-                       StorageType st, i64 nnz, std::vector<std::shared_ptr<void> >& keep) {
+                       StorageType st, bool tile_acc64, i64 nnz, std::vector<std::shared_ptr<void> >& keep) {
     hipStream_t s = ctx->stream;
     TiledLines t;
     t.n_major = n_major_out;
@@ -313,13 +314,15 @@ TiledLines build_lines(irlba_hip_context* ctx, const std::int32_t* d_idx, const 
     sval.reset();
     counts.reset();
 
-    // nnz-balanced chunks: enough workgroups to fill every CU once.
-    int target = ctx->cu_count;
+    // nnz-balanced chunks for exactly one resident wave of workgroups: as many
+    // blocks per CU as the LDS tile allows, times the CU count.
+    const size_t tile_bytes = static_cast<size_t>(t.tile_len) * (st == ST_F64 || tile_acc64 ? 8 : 4) + 1024;
+    const int per_cu = static_cast<int>(std::max<size_t>(1, std::min<size_t>(2, ctx->lds_per_cu / tile_bytes)));
+    int target = ctx->cu_count * per_cu;
     if (const char* e = std::getenv("IRLBA_HIP_SPMV_BLOCKS")) target = std::max(1, std::atoi(e));
-    t.G = static_cast<int>(std::max<i64>(1, std::min<i64>(std::max<i64>(n_major_out, 1),
-                                                          (target + t.n_tiles - 1) / t.n_tiles)));
-    auto chunk = dalloc<int>(static_cast<i64>(t.n_tiles) * (t.G + 1));
-    launch_chunk_bounds(ptr.get(), n_major_out, t.n_tiles, t.G, chunk.get(), s);
+    t.B = static_cast<int>(std::max<i64>(1, std::min<i64>(std::max<i64>(nseg, 1), target)));
+    auto chunk = dalloc<i64>(t.B + 1);
+    launch_chunk_bounds(ptr.get(), nseg, t.B, chunk.get(), s);
     IRLBA_HIP_CHECK(hipStreamSynchronize(s));
 
     t.ptr = ptr.get();
@@ -388,8 +391,10 @@ std::shared_ptr<SparseBase> make_sparse(irlba_hip_context* ctx, i64 nrow, i64 nc
     sp->column_major = column_major;
     const int tl = tile_elems(ctx, acc64);
     // by_col: lines are columns, the input (minor) dimension is rows.
-    sp->by_col = build_lines(ctx, d_idx, d_val, in_t, d_ptr, primary, !column_major, ncol, nrow, tl, st, nnz, sp->keep);
-    sp->by_row = build_lines(ctx, d_idx, d_val, in_t, d_ptr, primary, column_major, nrow, ncol, tl, st, nnz, sp->keep);
+    sp->by_col = build_lines(ctx, d_idx, d_val, in_t, d_ptr, primary, !column_major, ncol, nrow, tl, st, acc64, nnz,
+                             sp->keep);
+    sp->by_row = build_lines(ctx, d_idx, d_val, in_t, d_ptr, primary, column_major, nrow, ncol, tl, st, acc64, nnz,
+                             sp->keep);
     const size_t ab = acc64 ? 8 : 4;
     sp->partial_row = dalloc_bytes(ab * std::max<i64>(1, sp->by_row.n_tiles > 1 ? sp->by_row.n_tiles * nrow : 1));
     sp->partial_col = dalloc_bytes(ab * std::max<i64>(1, sp->by_col.n_tiles > 1 ? sp->by_col.n_tiles * ncol : 1));
@@ -678,7 +683,7 @@ struct Engine {
     hipStream_t s;
     i64 nr, nc, work;
     double eps;
-    std::shared_ptr<double> V, W, F, Wn, dnorm, partials, tvec, norm_part, Yt, rot_tmp;
+    std::shared_ptr<double> V, W, F, Wn, dnorm, partials, tvec, norm_part, Yt;
     int ldy = 0;
     std::mt19937_64 eng;
     std::vector<double> S, R;  // host copies of the sweep norms
@@ -698,7 +703,8 @@ struct Engine {
         tvec = dalloc<double>(work);
         norm_part = dalloc<double>(G);
         ldy = static_cast<int>((work + 15) / 16 * 16);
-        Yt = dalloc<double>(work * ldy);
+        Yt = dalloc<double>(2 * work * ldy);
+        for (auto& p : yt_pinned) IRLBA_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(double) * work * ldy));
         IRLBA_HIP_CHECK(hipMemsetAsync(W.get(), 0, sizeof(double) * nr * work, s));
         IRLBA_HIP_CHECK(hipMemsetAsync(V.get(), 0, sizeof(double) * nc * work, s));
         IRLBA_HIP_CHECK(hipMemsetAsync(dnorm.get(), 0, sizeof(double) * (2 * work + 4), s));
@@ -719,7 +725,7 @@ struct Engine {
             return;
         }
         launch_cgs_dots(M, L, static_cast<int>(n), f, L, partials.get(), s);
-        launch_cgs_reduce(partials.get(), cgs_blocks(L), static_cast<int>(n), tvec.get(), s);
+        launch_cgs_reduce(partials.get(), L, static_cast<int>(n), tvec.get(), s);
         launch_cgs_update(M, L, static_cast<int>(n), tvec.get(), f, L, norm_part.get(), s);
     }
     void norm_to(double* f, i64 L, double* norm_dev, double* dst) {
@@ -818,21 +824,25 @@ struct Engine {
         }
     }
 
-    // dst[:, :k] = src[:, :work] Y[:, :k] on the matrix cores.
-    void rotate(double* M, i64 L, const HostMat& Y, i64 k, double* out, i64 ld_out) {
-        std::vector<double> yt(static_cast<size_t>(work * ldy), 0.0);
+    // out[:, :k] = M[:, :work] Y[:, :k] on the matrix cores (out == M allowed).
+    // `slot` picks one of two pinned staging buffers, so the V and W rotations
+    // of one restart queue without a host round trip; a slot is rewritten only
+    // after the next sweep's synchronisation.
+    void rotate(int slot, double* M, i64 L, const HostMat& Y, i64 k, double* out, i64 ld_out) {
+        double* yt = yt_pinned[slot];
+        std::fill(yt, yt + work * ldy, 0.0);
         for (i64 r = 0; r < work; ++r)
             for (i64 c = 0; c < k; ++c) yt[r * ldy + c] = Y(r, c);
-        IRLBA_HIP_CHECK(hipMemcpyAsync(Yt.get(), yt.data(), sizeof(double) * yt.size(), hipMemcpyHostToDevice, s));
-        IRLBA_HIP_CHECK(hipStreamSynchronize(s));  // yt is a host temporary
-        if (out == M && k > 128) {
-            if (!rot_tmp) rot_tmp = dalloc<double>(std::max(nr, nc) * work);
-            launch_rotate(M, L, static_cast<int>(work), Yt.get(), ldy, static_cast<int>(k), rot_tmp.get(), L, L, s);
-            IRLBA_HIP_CHECK(hipMemcpyAsync(M, rot_tmp.get(), sizeof(double) * L * k, hipMemcpyDeviceToDevice, s));
-        } else {
-            launch_rotate(M, L, static_cast<int>(work), Yt.get(), ldy, static_cast<int>(k), out, ld_out, L, s);
-        }
-        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+        double* dY = Yt.get() + static_cast<i64>(slot) * work * ldy;
+        IRLBA_HIP_CHECK(hipMemcpyAsync(dY, yt, sizeof(double) * work * ldy, hipMemcpyHostToDevice, s));
+        launch_rotate(M, L, static_cast<int>(work), dY, ldy, static_cast<int>(k), out, ld_out, L, s);
+    }
+
+    double* yt_pinned[2] = {nullptr, nullptr};
+    ~Engine() {
+        (void)hipStreamSynchronize(s);
+        for (double* p : yt_pinned)
+            if (p) (void)hipHostFree(p);
     }
 };
 
@@ -928,9 +938,9 @@ std::pair<bool, int> run_compute(irlba_hip_context* ctx, Op& op, i64 number, con
         }
         prevS = BS;
         k = update_k(k, requested, n_conv, work);
-        eg.rotate(eg.V.get(), nc, BV, k, eg.V.get(), nc);
+        eg.rotate(0, eg.V.get(), nc, BV, k, eg.V.get(), nc);
         launch_scale_copy(eg.F.get(), nc, R_F, eg.Vc(k), s);
-        eg.rotate(eg.W.get(), nr, BU, k, eg.W.get(), nr);
+        eg.rotate(1, eg.W.get(), nr, BU, k, eg.W.get(), nr);
         std::fill(B.a.begin(), B.a.end(), 0.0);
         for (i64 l = 0; l < k; ++l) {
             B(l, l) = BS[l];
@@ -954,8 +964,11 @@ std::pair<bool, int> run_compute(irlba_hip_context* ctx, Op& op, i64 number, con
         oV = dV.get();
     }
     if (requested > 0) {
-        eg.rotate(eg.W.get(), nr, BU, requested, oU, nr);
-        eg.rotate(eg.V.get(), nc, BV, requested, oV, nc);
+        // After max_iterations the last restart's rotations may still be reading
+        // the pinned staging slots; finish them before the slots are rewritten.
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+        eg.rotate(0, eg.W.get(), nr, BU, requested, oU, nr);
+        eg.rotate(1, eg.V.get(), nc, BV, requested, oV, nc);
     }
     const auto kind = device_ptrs ? hipMemcpyHostToDevice : hipMemcpyHostToHost;
     IRLBA_HIP_CHECK(hipMemcpyAsync(D, BS.data(), sizeof(double) * requested, kind, s));
@@ -1053,6 +1066,7 @@ int irlba_hip_context_create(int device, irlba_hip_context** out) {
         IRLBA_HIP_CHECK(hipGetDeviceProperties(&prop, device));
         ctx->cu_count = prop.multiProcessorCount;
         ctx->lds_per_block = prop.sharedMemPerBlock;
+        ctx->lds_per_cu = std::max<size_t>(prop.maxSharedMemoryPerMultiProcessor, prop.sharedMemPerBlock);
         *out = ctx.release();
     });
 }
@@ -1324,7 +1338,7 @@ int irlba_hip_orthogonalize(irlba_hip_context* ctx, const double* M, int64_t row
         IRLBA_HIP_CHECK(hipMemcpy(df.get(), vec, sizeof(double) * rows, hipMemcpyHostToDevice));
         if (ncols > 0) {
             launch_cgs_dots(dM.get(), rows, static_cast<int>(ncols), df.get(), rows, part.get(), s);
-            launch_cgs_reduce(part.get(), G, static_cast<int>(ncols), t.get(), s);
+            launch_cgs_reduce(part.get(), rows, static_cast<int>(ncols), t.get(), s);
             launch_cgs_update(dM.get(), rows, static_cast<int>(ncols), t.get(), df.get(), rows, np.get(), s);
         } else {
             launch_sumsq(df.get(), rows, np.get(), s);

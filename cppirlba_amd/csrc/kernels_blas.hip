@@ -18,54 +18,79 @@ int cgs_blocks(i64 L) {
 static inline i64 cgs_chunk(i64 L, int G) { return (L + G - 1) / G; }
 
 // ------------------------------------------------------------------ t = M' f (partials)
-constexpr int kDotBatch = 8;
+// Thread per row (coalesced column reads, one f load per row), columns in
+// batches of 32 held in registers; the 32 per-lane partial sums are reduced
+// across the wave by recursive halving (lanes exchange the half of the batch
+// the partner keeps: 16+8+4+2+1 shuffles, then one xor-add), which leaves
+// column q of the batch in lanes 2q and 2q+1; four waves combine through LDS.
+constexpr int kDotCols = 32;
+
+int cgs_dot_blocks(i64 L) {
+    const i64 by_1k = (L + 1023) / 1024, by_256 = (L + 255) / 256;
+    return static_cast<int>(std::max<i64>(1, std::min<i64>(1024, std::max(by_1k, std::min<i64>(by_256, 256)))));
+}
 
 __global__ __launch_bounds__(256) void k_cgs_dots(const double* __restrict__ M, i64 ld, int n,
                                                   const double* __restrict__ f, i64 L, i64 chunk,
                                                   double* __restrict__ partials) {
+    __shared__ double red[4][kDotCols];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const i64 r0 = static_cast<i64>(blockIdx.x) * chunk;
     const i64 r1 = min(L, r0 + chunk);
-    // wave w owns columns w, w+4, w+8, ...; batches of kDotBatch columns share each f load.
-    for (int q0 = 0; w + 4 * q0 < n; q0 += kDotBatch) {
-        double acc[kDotBatch];
+    for (int cb = 0; cb < n; cb += kDotCols) {
+        double v[kDotCols];
 #pragma unroll
-        for (int q = 0; q < kDotBatch; ++q) acc[q] = 0;
-        for (i64 r = r0 + lane; r < r1; r += 64) {
+        for (int q = 0; q < kDotCols; ++q) v[q] = 0;
+        const int nb = min(kDotCols, n - cb);
+        for (i64 r = r0 + threadIdx.x; r < r1; r += 256) {
             const double fr = f[r];
+            const double* Mr = M + r + static_cast<i64>(cb) * ld;
 #pragma unroll
-            for (int q = 0; q < kDotBatch; ++q) {
-                const int i = w + 4 * (q0 + q);
-                if (i < n) acc[q] += M[r + i * ld] * fr;
+            for (int q = 0; q < kDotCols; ++q)
+                if (q < nb) v[q] += Mr[q * ld] * fr;
+        }
+#pragma unroll
+        for (int half = kDotCols / 2, o = 32; half >= 1; half >>= 1, o >>= 1) {
+            const bool upper = (lane & o) != 0;
+#pragma unroll
+            for (int j = 0; j < half; ++j) {
+                const double send = upper ? v[j] : v[j + half];
+                const double keep = upper ? v[j + half] : v[j];
+                v[j] = keep + __shfl_xor(send, o, 64);
             }
         }
-#pragma unroll
-        for (int q = 0; q < kDotBatch; ++q) {
-            const int i = w + 4 * (q0 + q);
-            const double v = wave_sum(acc[q]);
-            if (lane == 0 && i < n) partials[static_cast<i64>(blockIdx.x) * n + i] = v;
-        }
+        v[0] += __shfl_xor(v[0], 1, 64);
+        if ((lane & 1) == 0) red[w][lane >> 1] = v[0];
+        __syncthreads();
+        if (threadIdx.x < nb)
+            partials[static_cast<i64>(blockIdx.x) * n + cb + threadIdx.x] =
+                (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+        __syncthreads();
     }
 }
 
 void launch_cgs_dots(const double* M, i64 ld, int n, const double* f, i64 L, double* partials, hipStream_t s) {
     if (n <= 0) return;
-    const int G = cgs_blocks(L);
+    const int G = cgs_dot_blocks(L);
     hipLaunchKernelGGL(k_cgs_dots, dim3(G), dim3(256), 0, s, M, ld, n, f, L, cgs_chunk(L, G), partials);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
-__global__ void k_cgs_reduce(const double* __restrict__ partials, int G, int n, double* __restrict__ t) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double acc = 0;
-        for (int b = 0; b < G; ++b) acc += partials[static_cast<i64>(b) * n + i];
-        t[i] = acc;
-    }
+// t[i] = sum over blocks of the partials of column i: one block per column, a
+// fixed strided split and tree, so the result does not depend on timing.
+__global__ __launch_bounds__(256) void k_cgs_reduce(const double* __restrict__ partials, int G, int n,
+                                                    double* __restrict__ t) {
+    __shared__ double red[32];
+    const int i = blockIdx.x;
+    double acc = 0, dummy = 0;
+    for (int b = threadIdx.x; b < G; b += blockDim.x) acc += partials[static_cast<i64>(b) * n + i];
+    block_sum2(acc, dummy, red);
+    if (threadIdx.x == 0) t[i] = acc;
 }
 
-void launch_cgs_reduce(const double* partials, int G, int n, double* t, hipStream_t s) {
+void launch_cgs_reduce(const double* partials, i64 L, int n, double* t, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cgs_reduce, dim3(1), dim3(256), 0, s, partials, G, n, t);
+    hipLaunchKernelGGL(k_cgs_reduce, dim3(n), dim3(256), 0, s, partials, cgs_dot_blocks(L), n, t);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
@@ -222,65 +247,123 @@ void launch_scale_columns(double* M, i64 ld, i64 L, int k, const double* d, hipS
 
 // ------------------------------------------------------------------ restart rotation on MFMA
 // out[:, :k] = M[:, :work] * Y, Y given row-major as Yt[work][ldy] (ldy a multiple
-// of 16, zero padded).  Each wave computes a 16-row strip for up to
-// kRotTiles*16 columns with v_mfma_f64_16x16x4f64:
+// of 16, zero padded), k <= 128 per pass.  A wave owns S strips of 16 rows and
+// all NCT = ceil(k/16) column tiles: per K step of 4 it loads S A fragments
+// straight from HBM (each a 4-column x 16-row block = four full 128-byte
+// lines) and NCT B fragments (L1/L2-resident Yt), then issues S*NCT
+// v_mfma_f64_16x16x4f64; the next step's fragments are loaded before the
+// current MFMAs.  The finished tiles go through a per-wave LDS transpose so that
+// each store writes whole column segments.
+// MFMA operand layout (lane l, checked on gfx950):
 //   A operand: lane l holds M[row0 + l%16, kk + l/16]
 //   B operand: lane l holds Y[kk + l/16, col0 + l%16]
-//   D:         lane l, element i is out[row0 + l/16 + 4*i, col0 + l%16]  (checked on gfx950)
-// All reads of a strip precede its writes, so out == M is safe when one pass
-// covers every output column (k <= kRotTiles*16).
+//   D:         lane l, element i is out[row0 + l/16 + 4*i, col0 + l%16]
+// A wave reads all of its rows before it writes any, and waves own disjoint
+// rows, so out == M (the in-place restart update) is safe.
 typedef double double4_t __attribute__((ext_vector_type(4)));
-constexpr int kRotTiles = 8;
 
-__global__ __launch_bounds__(256) void k_rotate(const double* M, i64 ld, int work,
-                                                const double* __restrict__ Yt, int ldy, int k, int ct_begin,
-                                                double* out, i64 ld_out, i64 L) {
+template <int NCT, int S>
+__global__ __launch_bounds__(256) void k_rotate(const double* M, i64 ld, int work, const double* __restrict__ Yt,
+                                                int ldy, int k, int col_begin, double* out, i64 ld_out, i64 L) {
+    constexpr int ROWS = 16 * S;         // rows per wave
+    constexpr int LDR = ROWS + 1;        // padded LDS column stride (bank spread)
+    __shared__ double stage[4][16 * LDR];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const i64 row0 = (static_cast<i64>(blockIdx.x) * 4 + w) * 16;
-    if (row0 >= L) return;
-    const int nct = min(kRotTiles, (k + 15) / 16 - ct_begin);
-    double4_t acc[kRotTiles];
-#pragma unroll
-    for (int c = 0; c < kRotTiles; ++c) acc[c] = double4_t{0, 0, 0, 0};
     const int lr = lane & 15, lk = lane >> 4;
-    const i64 arow = row0 + lr;
-    for (int kk = 0; kk < work; kk += 4) {
-        const int kidx = kk + lk;
-        const double a = (arow < L && kidx < work) ? M[arow + static_cast<i64>(kidx) * ld] : 0.0;
+    const i64 row0 = (static_cast<i64>(blockIdx.x) * 4 + w) * ROWS;
+    if (row0 >= L) return;
+    const double* Yc = Yt + col_begin;
+
+    double4_t acc[S][NCT];
 #pragma unroll
-        for (int c = 0; c < kRotTiles; ++c) {
-            if (c < nct) {
-                const int col = (ct_begin + c) * 16 + lr;
-                const double b = kidx < work ? Yt[static_cast<i64>(kidx) * ldy + col] : 0.0;
-                acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) acc[s][c] = double4_t{0, 0, 0, 0};
+
+    auto load_a = [&](int kidx, double (&a)[S]) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const i64 row = row0 + s * 16 + lr;
+            a[s] = (kidx < work && row < L) ? M[row + static_cast<i64>(kidx) * ld] : 0.0;
+        }
+    };
+    auto load_b = [&](int kidx, double (&b)[NCT]) {
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) b[c] = kidx < work ? Yc[static_cast<i64>(kidx) * ldy + c * 16 + lr] : 0.0;
+    };
+
+    // A fragments are loaded PD steps ahead (a ring of PD register sets), so a
+    // wave keeps PD*S*512 bytes of HBM reads in flight at 1-2 waves per SIMD.
+    constexpr int PD = 16 / S;
+    double abuf[PD][S];
+#pragma unroll
+    for (int p = 0; p < PD; ++p) load_a(4 * p + lk, abuf[p]);
+    for (int kk0 = 0; kk0 < work; kk0 += 4 * PD) {
+#pragma unroll
+        for (int p = 0; p < PD; ++p) {
+            const int kk = kk0 + 4 * p;
+            if (kk < work) {
+                double b[NCT];
+                load_b(kk + lk, b);
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+#pragma unroll
+                    for (int c = 0; c < NCT; ++c)
+                        acc[s][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(abuf[p][s], b[c], acc[s][c], 0, 0, 0);
+                load_a(kk + 4 * PD + lk, abuf[p]);
             }
         }
     }
+
+    double* st = stage[w];
+    constexpr int COLS_PER_STORE = 64 / ROWS;  // columns one store instruction covers
 #pragma unroll
-    for (int c = 0; c < kRotTiles; ++c) {
-        if (c < nct) {
-            const int col = (ct_begin + c) * 16 + lr;
-            if (col < k) {
+    for (int c = 0; c < NCT; ++c) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const i64 row = row0 + lk + 4 * i;
-                    if (row < L) out[row + static_cast<i64>(col) * ld_out] = acc[c][i];
-                }
-            }
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) st[lr * LDR + s * 16 + lk + 4 * i] = acc[s][c][i];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the LDS writes have landed
+        const int sub = lane / ROWS, rr = lane - sub * ROWS;
+        const i64 row = row0 + rr;
+#pragma unroll
+        for (int cc = 0; cc < 16; cc += COLS_PER_STORE) {
+            const int col = c * 16 + cc + sub;
+            if (col_begin + col < k && row < L) out[row + static_cast<i64>(col_begin + col) * ld_out] = st[(cc + sub) * LDR + rr];
         }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_s_waitcnt(0xc07f);
     }
+}
+
+template <int NCT>
+static void rotate_launch(const double* M, i64 ld, int work, const double* Yt, int ldy, int k, int col_begin,
+                          double* out, i64 ld_out, i64 L, hipStream_t s) {
+    constexpr int S = NCT <= 4 ? 4 : 2;
+    const i64 rows_per_block = 4 * 16 * S;
+    const i64 grid = (L + rows_per_block - 1) / rows_per_block;
+    hipLaunchKernelGGL((k_rotate<NCT, S>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, M, ld, work, Yt, ldy, k,
+                       col_begin, out, ld_out, L);
+    IRLBA_HIP_LAUNCH_CHECK();
 }
 
 void launch_rotate(const double* M, i64 ld, int work, const double* Yt, int ldy, int k, double* out, i64 ld_out,
                    i64 L, hipStream_t s) {
     if (k <= 0 || L <= 0) return;
-    const int tiles = (k + 15) / 16;
-    if (out == M && tiles > kRotTiles) throw std::runtime_error("in-place rotation limited to 128 columns");
-    const i64 grid = (L + 63) / 64;
-    for (int ct = 0; ct < tiles; ct += kRotTiles) {
-        hipLaunchKernelGGL(k_rotate, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, M, ld, work, Yt, ldy, k, ct,
-                           out, ld_out, L);
-        IRLBA_HIP_LAUNCH_CHECK();
+    if (out == M && k > 128) throw std::runtime_error("in-place rotation covers at most 128 columns per pass");
+    for (int cb = 0; cb < k; cb += 128) {
+        const int nct = std::min(8, (k - cb + 15) / 16);
+        switch (nct) {
+            case 1: rotate_launch<1>(M, ld, work, Yt, ldy, k, cb, out, ld_out, L, s); break;
+            case 2: rotate_launch<2>(M, ld, work, Yt, ldy, k, cb, out, ld_out, L, s); break;
+            case 3: rotate_launch<3>(M, ld, work, Yt, ldy, k, cb, out, ld_out, L, s); break;
+            case 4: rotate_launch<4>(M, ld, work, Yt, ldy, k, cb, out, ld_out, L, s); break;
+            case 5: rotate_launch<5>(M, ld, work, Yt, ldy, k, cb, out, ld_out, L, s); break;
+            case 6: rotate_launch<6>(M, ld, work, Yt, ldy, k, cb, out, ld_out, L, s); break;
+            case 7: rotate_launch<7>(M, ld, work, Yt, ldy, k, cb, out, ld_out, L, s); break;
+            default: rotate_launch<8>(M, ld, work, Yt, ldy, k, cb, out, ld_out, L, s); break;
+        }
     }
 }
 

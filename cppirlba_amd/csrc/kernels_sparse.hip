@@ -278,24 +278,25 @@ void launch_sort_segments(const i64* ptr, const unsigned long long* counts, i64 
 }
 
 // ------------------------------------------------------------------ nnz-balanced chunks
-__global__ void k_chunk_bounds(const i64* __restrict__ ptr, i64 n_major, int n_tiles, int G, int* chunk) {
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n_tiles * (G + 1)) return;
-    const int t = id / (G + 1), g = id % (G + 1);
-    const i64* P = ptr + static_cast<i64>(t) * n_major;
-    const i64 total = P[n_major] - P[0];
-    const i64 target = P[0] + (g == G ? total : (total / G) * g + ((total % G) * g) / G);
-    i64 lo = 0, hi = n_major;  // smallest m with P[m] >= target
+// The (tile, line) segments are one tile-major sequence whose offsets `ptr`
+// increase; block b of a product owns segments [chunk[b], chunk[b+1]), cut at
+// the nnz quantiles b*total/B.  B is one resident wave of blocks, so every CU
+// gets the same number of nonzeros and no block waits for a second wave.
+__global__ void k_chunk_bounds(const i64* __restrict__ ptr, i64 nseg, int B, i64* chunk) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > B) return;
+    const i64 total = ptr[nseg];
+    const i64 target = (total / B) * b + ((total % B) * b) / B;
+    i64 lo = 0, hi = nseg;  // smallest s with ptr[s] >= target
     while (lo < hi) {
         const i64 mid = (lo + hi) / 2;
-        if (P[mid] < target) lo = mid + 1; else hi = mid;
+        if (ptr[mid] < target) lo = mid + 1; else hi = mid;
     }
-    chunk[id] = static_cast<int>(g == G ? n_major : (g == 0 ? 0 : lo));
+    chunk[b] = b == B ? nseg : (b == 0 ? 0 : lo);
 }
 
-void launch_chunk_bounds(const i64* ptr, i64 n_major, int n_tiles, int G, int* chunk, hipStream_t s) {
-    const int n = n_tiles * (G + 1);
-    hipLaunchKernelGGL(k_chunk_bounds, dim3((n + 255) / 256), dim3(256), 0, s, ptr, n_major, n_tiles, G, chunk);
+void launch_chunk_bounds(const i64* ptr, i64 nseg, int B, i64* chunk, hipStream_t s) {
+    hipLaunchKernelGGL(k_chunk_bounds, dim3((B + 1 + 255) / 256), dim3(256), 0, s, ptr, nseg, B, chunk);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
@@ -305,9 +306,9 @@ struct SpmvParams {
     const i64* ptr;
     const std::uint16_t* idx;
     const void* val;
-    const int* chunk;
+    const i64* chunk;
     i64 n_major, n_minor;
-    int tile_len, n_tiles, G;
+    int tile_len, n_tiles, B;
     Prologue pro;
     Epilogue epi;
     AT* partial;
@@ -353,14 +354,25 @@ __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
     AT* xs = reinterpret_cast<AT*>(smem_raw);
     __shared__ double red[2 * 16];
 
-    const int t = blockIdx.x / p.G;
-    const int g = blockIdx.x - t * p.G;
-    const i64 base = static_cast<i64>(t) * p.tile_len;
+    const i64 S0 = p.chunk[blockIdx.x], S1 = p.chunk[blockIdx.x + 1];
+    if (S0 >= S1) return;  // uniform across the block
+    const i64 t_first = S0 / p.n_major, t_last = (S1 - 1) / p.n_major;
+    const VT* __restrict__ val = static_cast<const VT*>(p.val);
+    const std::uint16_t* __restrict__ ix = p.idx;
+    const int lane_in = threadIdx.x & (LPL - 1);
+    const int grp = threadIdx.x / LPL, ngrp = blockDim.x / LPL;
+    constexpr int STRIDE = LPL * VEC;
+    const Prologue& pro = p.pro;
+    const bool single = p.n_tiles == 1;
+    const double coef = (single && p.epi.axpy_src) ? *p.epi.axpy_coef : 0.0;
+
+    for (i64 t = t_first; t <= t_last; ++t) {
+    if (t > t_first) __syncthreads();  // every group is done with the previous tile
+    const i64 base = t * p.tile_len;
     const int cnt = static_cast<int>(min(static_cast<i64>(p.tile_len), p.n_minor - base));
 
     // Stage the input tile (scaled) in LDS; accumulate the rank-1 dot products.
     double g0 = 0, g1 = 0;
-    const Prologue& pro = p.pro;
     for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
         const double xv = pro.x[base + j];
         const double xsv = pro.xscale ? xv * pro.xscale[base + j] : xv;
@@ -373,22 +385,20 @@ __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
     } else {
         __syncthreads();
     }
-    if (p.n_tiles > 1 && g == 0 && threadIdx.x == 0) {
+    const i64 seg0 = t * p.n_major;  // the tile's first segment
+    if (p.n_tiles > 1 && S0 <= seg0 && seg0 < S1 && threadIdx.x == 0) {
         p.gamma_part[2 * t] = g0;
         p.gamma_part[2 * t + 1] = g1;
     }
     const double gamma[2] = {g0, g1};
 
-    const VT* __restrict__ val = static_cast<const VT*>(p.val);
-    const std::uint16_t* __restrict__ ix = p.idx;
-    const i64* __restrict__ P = p.ptr + static_cast<i64>(t) * p.n_major;
-    const int L0 = p.chunk[t * (p.G + 1) + g], L1 = p.chunk[t * (p.G + 1) + g + 1];
-    const int lane_in = threadIdx.x & (LPL - 1);
-    const int grp = threadIdx.x / LPL, ngrp = blockDim.x / LPL;
-    constexpr int STRIDE = LPL * VEC;
+    const i64* __restrict__ P = p.ptr + seg0;
+    const i64 L0 = max(S0, seg0) - seg0, L1 = min(S1, seg0 + p.n_major) - seg0;
 
-    for (int m = L0 + grp; m < L1; m += ngrp) {
+    for (i64 m = L0 + grp; m < L1; m += ngrp) {
         const i64 s0 = P[m], s1 = P[m + 1];
+        EpiOperands eop;
+        if (single && lane_in == 0) eop = load_epilogue(p.epi, m);  // in flight during the stream
         AT acc0 = 0, acc1 = 0;
         i64 s = s0 + lane_in * VEC;
         for (; s + 3 * STRIDE < s1; s += 4 * STRIDE) {
@@ -413,9 +423,10 @@ __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
         }
         AT acc = group_sum<LPL>(acc0 + acc1);
         if (lane_in == 0) {
-            if (p.n_tiles == 1) p.epi.out[m] = apply_epilogue(p.epi, m, static_cast<double>(acc), gamma);
-            else p.partial[static_cast<i64>(t) * p.n_major + m] = acc;
+            if (single) p.epi.out[m] = finish_epilogue(p.epi, eop, static_cast<double>(acc), gamma, coef);
+            else p.partial[seg0 + m] = acc;
         }
+    }
     }
 }
 
@@ -447,14 +458,14 @@ static void spmv_launch_one(const TiledLines& tl, const Prologue& pro, const Epi
     p.n_minor = tl.n_minor;
     p.tile_len = tl.tile_len;
     p.n_tiles = tl.n_tiles;
-    p.G = tl.G;
+    p.B = tl.B;
     p.pro = pro;
     p.epi = epi;
     p.partial = static_cast<AT*>(partial);
     p.gamma_part = gamma_part;
     const size_t lds = static_cast<size_t>(tl.tile_len) * sizeof(AT);
     if (ev0) IRLBA_HIP_CHECK(hipEventRecord(ev0, s));
-    hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, VEC>), dim3(tl.G * tl.n_tiles), dim3(1024), lds, s, p);
+    hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, VEC>), dim3(tl.B), dim3(1024), lds, s, p);
     IRLBA_HIP_LAUNCH_CHECK();
     if (ev1) IRLBA_HIP_CHECK(hipEventRecord(ev1, s));
     if (tl.n_tiles > 1) {
