@@ -336,6 +336,48 @@ TiledLines build_lines(irlba_hip_context* ctx, const std::int32_t* d_idx, const 
     return t;
 }
 
+// Lanes per line: the best group width depends on the segment lengths, the
+// number of tiles and the precision, so each orientation times a plain product
+// at 8, 16, 32 and 64 lanes (HIP events, best of two) and keeps the fastest.
+void autotune_lanes(irlba_hip_context* ctx, TiledLines& t, bool acc64, void* partial, double* gamma_part) {
+    if (std::getenv("IRLBA_HIP_LPL") || t.nnz == 0 || t.n_major == 0) return;
+    if (t.nnz < (static_cast<i64>(1) << 20)) return;  // small matrices: launch-bound, keep the heuristic
+    hipStream_t s = ctx->stream;
+    auto x = dalloc<double>(t.n_minor);
+    auto y = dalloc<double>(t.n_major);
+    IRLBA_HIP_CHECK(hipMemsetAsync(x.get(), 0, sizeof(double) * t.n_minor, s));
+    Prologue pro;
+    pro.x = x.get();
+    Epilogue epi;
+    epi.out = y.get();
+    hipEvent_t e0, e1;
+    IRLBA_HIP_CHECK(hipEventCreate(&e0));
+    IRLBA_HIP_CHECK(hipEventCreate(&e1));
+    float best = 0;
+    int best_lpl = t.lpl;
+    for (int lpl : {8, 16, 32, 64}) {
+        t.lpl = lpl;
+        launch_spmv_tiled(t, pro, epi, acc64, partial, gamma_part, s);  // warm-up
+        float ms_best = 0;
+        for (int rep = 0; rep < 2; ++rep) {
+            IRLBA_HIP_CHECK(hipEventRecord(e0, s));
+            launch_spmv_tiled(t, pro, epi, acc64, partial, gamma_part, s);
+            IRLBA_HIP_CHECK(hipEventRecord(e1, s));
+            IRLBA_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            IRLBA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (rep == 0 || ms < ms_best) ms_best = ms;
+        }
+        if (best == 0 || ms_best < best) {
+            best = ms_best;
+            best_lpl = lpl;
+        }
+    }
+    t.lpl = best_lpl;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
 int tile_elems(irlba_hip_context* ctx, bool acc64) {
     size_t budget = std::min<size_t>(128 * 1024, ctx->lds_per_block > 8192 ? ctx->lds_per_block - 4096 : 4096);
     if (const char* e = std::getenv("IRLBA_HIP_TILE_BYTES")) budget = std::min<size_t>(budget, std::atol(e));
@@ -400,6 +442,11 @@ std::shared_ptr<SparseBase> make_sparse(irlba_hip_context* ctx, i64 nrow, i64 nc
     sp->partial_col = dalloc_bytes(ab * std::max<i64>(1, sp->by_col.n_tiles > 1 ? sp->by_col.n_tiles * ncol : 1));
     sp->gamma_row = dalloc<double>(2 * sp->by_row.n_tiles);
     sp->gamma_col = dalloc<double>(2 * sp->by_col.n_tiles);
+    u_idx.reset();  // the raw upload is no longer needed
+    u_val.reset();
+    u_ptr.reset();
+    autotune_lanes(ctx, sp->by_row, acc64, sp->partial_row.get(), sp->gamma_row.get());
+    autotune_lanes(ctx, sp->by_col, acc64, sp->partial_col.get(), sp->gamma_col.get());
     return sp;
 }
 
@@ -431,22 +478,31 @@ void host_svd(const HostMat& A, HostMat& U, std::vector<double>& S, HostMat& V) 
     HostMat G = A, Q(n, n);
     for (i64 i = 0; i < n; ++i) Q(i, i) = 1;
     std::vector<double> sq(n);
-    auto colsq = [&](i64 j) {
-        const double* g = G.col(j);
-        double s = 0;
-        for (i64 r = 0; r < m; ++r) s += g[r] * g[r];
-        return s;
+    // Dot product with four interleaved partial sums (a fixed order, so the
+    // result is reproducible) to break the serial add chain.
+    auto dotm = [m](const double* x, const double* y) {
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        i64 r = 0;
+        for (; r + 3 < m; r += 4) {
+            s0 += x[r] * y[r];
+            s1 += x[r + 1] * y[r + 1];
+            s2 += x[r + 2] * y[r + 2];
+            s3 += x[r + 3] * y[r + 3];
+        }
+        for (; r < m; ++r) s0 += x[r] * y[r];
+        return (s0 + s1) + (s2 + s3);
     };
-    for (i64 j = 0; j < n; ++j) sq[j] = colsq(j);
     const double eps = std::numeric_limits<double>::epsilon();
     for (int sweep = 0; sweep < 80; ++sweep) {
+        // Exact squared norms at the start of each sweep; within the sweep they
+        // follow the rotations in closed form (alpha -= t*gamma, beta += t*gamma).
+        for (i64 j = 0; j < n; ++j) sq[j] = dotm(G.col(j), G.col(j));
         double worst = 0;
         for (i64 p = 0; p < n - 1; ++p) {
             for (i64 q = p + 1; q < n; ++q) {
                 double* gp = G.col(p);
                 double* gq = G.col(q);
-                double g = 0;
-                for (i64 r = 0; r < m; ++r) g += gp[r] * gq[r];
+                const double g = dotm(gp, gq);
                 const double scale = std::sqrt(sq[p] * sq[q]);
                 if (g == 0 || std::abs(g) <= eps * scale) continue;
                 worst = std::max(worst, std::abs(g) / scale);
@@ -465,12 +521,13 @@ void host_svd(const HostMat& A, HostMat& U, std::vector<double>& S, HostMat& V) 
                     qp[r] = c * x - s * y;
                     qq[r] = s * x + c * y;
                 }
-                sq[p] = colsq(p);
-                sq[q] = colsq(q);
+                sq[p] -= t * g;
+                sq[q] += t * g;
             }
         }
         if (worst <= eps) break;
     }
+    auto colsq = [&](i64 j) { return dotm(G.col(j), G.col(j)); };
     std::vector<i64> order(n);
     for (i64 j = 0; j < n; ++j) order[j] = j;
     std::vector<double> nrm(n);

@@ -25,6 +25,8 @@ static inline i64 cgs_chunk(i64 L, int G) { return (L + G - 1) / G; }
 // column q of the batch in lanes 2q and 2q+1; four waves combine through LDS.
 constexpr int kDotCols = 32;
 
+// ~1024-row chunks for long vectors (up to 1024 blocks), 256-row chunks for
+// short ones so that at least a few hundred blocks share the work.
 int cgs_dot_blocks(i64 L) {
     const i64 by_1k = (L + 1023) / 1024, by_256 = (L + 255) / 256;
     return static_cast<int>(std::max<i64>(1, std::min<i64>(1024, std::max(by_1k, std::min<i64>(by_256, 256)))));

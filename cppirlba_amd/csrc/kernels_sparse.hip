@@ -395,8 +395,19 @@ __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
     const i64* __restrict__ P = p.ptr + seg0;
     const i64 L0 = max(S0, seg0) - seg0, L1 = min(S1, seg0 + p.n_major) - seg0;
 
+    // The segment bounds of a group's next line are loaded while the current
+    // line streams, so a line never starts with an exposed pointer load.
+    i64 nxt0 = 0, nxt1 = 0;
+    if (L0 + grp < L1) {
+        nxt0 = P[L0 + grp];
+        nxt1 = P[L0 + grp + 1];
+    }
     for (i64 m = L0 + grp; m < L1; m += ngrp) {
-        const i64 s0 = P[m], s1 = P[m + 1];
+        const i64 s0 = nxt0, s1 = nxt1;
+        if (m + ngrp < L1) {
+            nxt0 = P[m + ngrp];
+            nxt1 = P[m + ngrp + 1];
+        }
         EpiOperands eop;
         if (single && lane_in == 0) eop = load_epilogue(p.epi, m);  // in flight during the stream
         AT acc0 = 0, acc1 = 0;
