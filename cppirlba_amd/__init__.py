@@ -468,3 +468,74 @@ def rotate(M, Y, ctx=None):
     check(lib().irlba_hip_rotate(ctx.handle, M.ctypes.data, M.shape[0], M.shape[1], Y.ctypes.data, Y.shape[1],
                                  out.ctypes.data))
     return out
+
+
+class SvdBackward:
+    """Factored gradient dL/dA = X Y' of a top-k SVD (see svd_backward)."""
+
+    def __init__(self, X, Y, grad_center, cg_iterations):
+        self.X = X
+        self.Y = Y
+        self.grad_center = grad_center
+        self.cg_iterations = cg_iterations
+
+    def dense(self):
+        return self.X @ self.Y.T
+
+
+_COMPLEMENT = {"none": 0, "rank_k": 1, "exact": 2}
+
+
+def svd_backward(matrix, U, D, V, grad_D=None, grad_U=None, grad_V=None, complement="exact",
+                 max_cg_iterations=1000, cg_tolerance=1e-12, grad_center=False):
+    """Backward of the top-k SVD of ``matrix`` (U rows x k, D k, V cols x k).
+
+    Returns dL/dA for L with gradients grad_D, grad_U, grad_V (None = zero) as
+    ``SvdBackward`` (X rows x 2k, Y cols x 2k).  ``complement``: "none",
+    "rank_k" (exact when the matrix has rank k) or "exact" (CG solves through the
+    matrix's own products).  ``grad_center``: also return -Y (X' 1), the gradient
+    with respect to a column-centre vector subtracted from the matrix.
+    """
+    nr, nc = matrix.shape
+    U = _colmajor(U)
+    V = _colmajor(V)
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    k = D.size
+    if U.shape != (nr, k) or V.shape != (nc, k):
+        raise ValueError("U, D, V do not match the matrix")
+
+    def opt(a, shape):
+        if a is None:
+            return None
+        a = _colmajor(a)
+        if a.shape != shape and not (len(shape) == 1 and a.size == shape[0]):
+            raise ValueError("gradient shape does not match")
+        return a
+
+    gD, gU, gV = opt(grad_D, (k,)), opt(grad_U, (nr, k)), opt(grad_V, (nc, k))
+    X = np.zeros((nr, 2 * k), order="F")
+    Y = np.zeros((nc, 2 * k), order="F")
+    gc = np.zeros(nc) if grad_center else None
+    its = ctypes.c_int32()
+    p = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+    check(lib().irlba_hip_svd_backward(matrix.handle, k, p(U), p(D), p(V), p(gD), p(gU), p(gV),
+                                       _COMPLEMENT[complement], int(max_cg_iterations), float(cg_tolerance),
+                                       p(X), p(Y), p(gc), ctypes.byref(its), 0))
+    return SvdBackward(X, Y, gc, its.value)
+
+
+def sddmm(nrow, ncol, indices, pointers, X, Y, column_major=True, ctx=None):
+    """out[e] = X[row_e] . Y[col_e] over the stored entries of a compressed pattern."""
+    ctx = ctx or default_context()
+    X = _colmajor(X)
+    Y = _colmajor(Y)
+    if X.shape[0] != nrow or Y.shape[0] != ncol or X.shape[1] != Y.shape[1]:
+        raise ValueError("factor shapes do not match the pattern")
+    idx = np.ascontiguousarray(indices, dtype=np.int32)
+    ptr = np.ascontiguousarray(pointers, dtype=np.int64)
+    if ptr.size != (ncol if column_major else nrow) + 1:
+        raise ValueError("pointers have the wrong length")
+    out = np.zeros(idx.size)
+    check(lib().irlba_hip_sddmm(ctx.handle, nrow, ncol, idx.ctypes.data, ptr.ctypes.data, int(column_major),
+                                X.shape[1], X.ctypes.data, Y.ctypes.data, out.ctypes.data, 0))
+    return out

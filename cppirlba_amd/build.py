@@ -24,6 +24,7 @@ SOURCES = [
     "kernels_dense.hip",
     "kernels_blas.hip",
     "kernels_generate.hip",
+    "kernels_backward.hip",
 ]
 
 COMMON = [

@@ -200,6 +200,12 @@ void launch_vec_combine(double* out, const double* a, const double* b, i64 n, in
 void launch_axpy_dev(double* y, const double* src, const double* coef, i64 n, hipStream_t s);
 void launch_scale_columns(double* M, i64 ld, i64 L, int k, const double* d, hipStream_t s);
 
+// Differentiable SVD (kernels_backward.hip).
+void launch_to_row_major(const double* X, i64 rows, i64 ld, int R, double* out, hipStream_t s);
+void launch_sddmm(const std::int32_t* idx, const i64* ptr, i64 n_lines, const double* line_f, const double* entry_f_rm,
+                  int R, double* out, hipStream_t s);
+void launch_axpby(double* out, const double* a, double alpha, const double* b, double beta, i64 n, hipStream_t s);
+
 // Synthetic data (counter-based, identical to the oracle's generator).
 void launch_gen_nb_count(i64 n_minor, i64 n_major, std::uint32_t thr, std::uint64_t seed, i64* counts,
                          hipStream_t s);

@@ -1086,6 +1086,211 @@ irlba_hip_matrix* decorate_scale(const irlba_hip_matrix* in, const std::shared_p
     return m;
 }
 
+// ------------------------------------------------------------------ differentiable top-k SVD
+// dL/dA for L = sum_i w_i s_i + <G_U, U> + <G_V, V> as X Y' with
+//   X = [U M + Z' | U],  Y = [V | Y'],
+//   M = diag(w) + (F o (J - J')) S + S (F o (K - K')),  J = U'G_U, K = V'G_V,
+//   F_ij = 1 / (s_j^2 - s_i^2) (i != j).
+// Complement columns (the part of dU, dV outside the top-k subspaces):
+//   mode 1: Z' = (G_U - U J) S^-1, Y' = (G_V - V K) S^-1 (exact when rank(A) = k);
+//   mode 2: Z'_i = s_i z_i + A t_i, Y'_i = A' z_i + s_i t_i with
+//           (s_i^2 I - P A A' P) z_i = P g_i on P = I - U U' and
+//           (s_i^2 I - Q A'A Q) t_i = Q h_i on Q = I - V V', solved by CG
+//           (positive definite when s_k > s_{k+1}).
+// Projections are applied as y - U (U'y) with the Gram-Schmidt kernels; no
+// dense I - UU' is formed.
+struct Backward {
+    irlba_hip_context* ctx;
+    const irlba_hip_matrix* m;
+    hipStream_t s;
+    i64 nr, nc, k;
+    std::shared_ptr<double> partials, tvec, norm_part, scal;
+
+    Backward(irlba_hip_context* c, const irlba_hip_matrix* mm, i64 k_)
+        : ctx(c), m(mm), s(c->stream), nr(mm->base->nr), nc(mm->base->nc), k(k_) {
+        const int G = std::max(cgs_blocks(nr), cgs_blocks(nc));
+        partials = dalloc<double>(static_cast<i64>(G) * std::max<i64>(2 * k, 1));
+        tvec = dalloc<double>(std::max<i64>(2 * k, 1));
+        norm_part = dalloc<double>(G);
+        scal = dalloc<double>(4);
+    }
+
+    // out[0..n) = M[:, :n]' f on the device.
+    void dots(const double* M, i64 L, i64 n, const double* f, double* out) {
+        launch_cgs_dots(M, L, static_cast<int>(n), f, L, partials.get(), s);
+        launch_cgs_reduce(partials.get(), L, static_cast<int>(n), out, s);
+    }
+    double dot(const double* a, const double* b, i64 L) {
+        dots(a, L, 1, b, scal.get());
+        double h = 0;
+        IRLBA_HIP_CHECK(hipMemcpyAsync(&h, scal.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+        return h;
+    }
+    // y -= M[:, :n] (M[:, :n]' y)
+    void project(const double* M, i64 L, i64 n, double* y) {
+        launch_cgs_dots(M, L, static_cast<int>(n), y, L, partials.get(), s);
+        launch_cgs_reduce(partials.get(), L, static_cast<int>(n), tvec.get(), s);
+        launch_cgs_update(M, L, static_cast<int>(n), tvec.get(), y, L, norm_part.get(), s);
+    }
+    // CG for (sig2 I - P B P) x = b on the complement of `basis` (L x k, P =
+    // I - basis basis'), with B = A A' (left, L = rows) or A' A (right, L = cols);
+    // b lies in the complement and so do all iterates.
+    int cg(bool left, double sig2, const double* basis, const double* b, double* x, int max_it, double tol) {
+        const i64 L = left ? nr : nc, Lo = left ? nc : nr;
+        auto r = dalloc<double>(L), p = dalloc<double>(L), ap = dalloc<double>(L), tmp = dalloc<double>(Lo);
+        auto op = [&](const double* in, double* out) {
+            apply_handle(*m, left, in, tmp.get(), nullptr, nullptr);    // A' in (left) or A in (right)
+            apply_handle(*m, !left, tmp.get(), out, nullptr, nullptr);  // A A' in or A' A in
+            project(basis, L, k, out);
+            launch_axpby(out, in, sig2, out, -1.0, L, s);               // sig2 in - P B in
+        };
+        launch_axpby(x, b, 1.0 / sig2, nullptr, 0.0, L, s);  // start from the rank-k answer
+        op(x, ap.get());
+        launch_axpby(r.get(), b, 1.0, ap.get(), -1.0, L, s);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(p.get(), r.get(), sizeof(double) * L, hipMemcpyDeviceToDevice, s));
+        const double bnorm = std::sqrt(dot(b, b, L));
+        double rr = dot(r.get(), r.get(), L);
+        int it = 0;
+        while (it < max_it && std::sqrt(rr) > tol * bnorm) {
+            op(p.get(), ap.get());
+            const double pap = dot(p.get(), ap.get(), L);
+            if (!(pap > 0)) throw std::runtime_error("complement system is not positive definite (needs s_k > s_{k+1})");
+            const double alpha = rr / pap;
+            launch_axpby(x, x, 1.0, p.get(), alpha, L, s);
+            launch_axpby(r.get(), r.get(), 1.0, ap.get(), -alpha, L, s);
+            const double rr_new = dot(r.get(), r.get(), L);
+            launch_axpby(p.get(), r.get(), 1.0, p.get(), rr_new / rr, L, s);
+            rr = rr_new;
+            ++it;
+        }
+        project(basis, L, k, x);  // drop the rounding drift back into span(basis)
+        return it;
+    }
+};
+
+int run_svd_backward(const irlba_hip_matrix* m, i64 k, const double* U, const double* sv, const double* V,
+                     const double* w, const double* GU, const double* GV, int complement, int max_cg, double cg_tol,
+                     double* X, double* Y, double* dcenter, bool device_ptrs) {
+    irlba_hip_context* ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    Backward bw(ctx, m, k);
+    const i64 nr = bw.nr, nc = bw.nc, R = 2 * k;
+    require(k > 0 && k <= std::min(nr, nc) && k <= 64, "k must lie in [1, min(rows, cols)] and be at most 64");
+    const auto in_kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    auto up = [&](const double* src, i64 n) {
+        auto d = dalloc<double>(n);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(d.get(), src, sizeof(double) * n, in_kind, s));
+        return d;
+    };
+    auto dU = up(U, nr * k), dV = up(V, nc * k), dGU = up(GU, nr * k), dGV = up(GV, nc * k);
+    std::vector<double> hs(k), hw(k);
+    if (device_ptrs) {
+        IRLBA_HIP_CHECK(hipMemcpyAsync(hs.data(), sv, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        IRLBA_HIP_CHECK(hipMemcpyAsync(hw.data(), w, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    } else {
+        std::copy(sv, sv + k, hs.begin());
+        std::copy(w, w + k, hw.begin());
+    }
+    // J = U'G_U, K = V'G_V (k x k, column i = basis' g_i).
+    auto dJ = dalloc<double>(k * k), dK = dalloc<double>(k * k);
+    for (i64 i = 0; i < k; ++i) {
+        bw.dots(dU.get(), nr, k, dGU.get() + i * nr, dJ.get() + i * k);
+        bw.dots(dV.get(), nc, k, dGV.get() + i * nc, dK.get() + i * k);
+    }
+    std::vector<double> J(k * k), K(k * k);
+    IRLBA_HIP_CHECK(hipMemcpyAsync(J.data(), dJ.get(), sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
+    IRLBA_HIP_CHECK(hipMemcpyAsync(K.data(), dK.get(), sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
+    IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+    for (i64 i = 0; i < k; ++i)
+        require(hs[i] > 0 && (i == 0 || hs[i] <= hs[i - 1]), "singular values must be positive and non-increasing");
+    // M on the host.
+    HostMat M(k, k);
+    for (i64 j = 0; j < k; ++j)
+        for (i64 i = 0; i < k; ++i) {
+            if (i == j) {
+                M(i, i) = hw[i];
+                continue;
+            }
+            const double den = hs[j] * hs[j] - hs[i] * hs[i];
+            const double F = den != 0 ? 1.0 / den : 0.0;  // repeated values: the gradient is undefined; drop the term
+            const double ju = J[i + j * k] - J[j + i * k], kv = K[i + j * k] - K[j + i * k];
+            M(i, j) = F * ju * hs[j] + hs[i] * F * kv;
+        }
+    auto dX = dalloc<double>(nr * R), dY = dalloc<double>(nc * R);
+    // out (L x cols) = A[:, :Ym.nr] Ym on the matrix cores (the restart-rotation kernel).
+    auto rotate_into = [&](const double* A, i64 L, const HostMat& Ym, double* out) {
+        const i64 inner = Ym.nr, cols = Ym.nc;
+        const int ldy = static_cast<int>((cols + 15) / 16 * 16);
+        std::vector<double> yt(static_cast<size_t>(inner * ldy), 0.0);
+        for (i64 r = 0; r < inner; ++r)
+            for (i64 c = 0; c < cols; ++c) yt[r * ldy + c] = Ym(r, c);
+        auto dYt = dalloc<double>(inner * ldy);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(dYt.get(), yt.data(), sizeof(double) * yt.size(), hipMemcpyHostToDevice, s));
+        launch_rotate(A, L, static_cast<int>(inner), dYt.get(), ldy, static_cast<int>(cols), out, L, L, s);
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));  // before the staging buffers go out of scope
+    };
+    // X[:, :k] = U M, X[:, k:] = U; Y[:, :k] = V.
+    rotate_into(dU.get(), nr, M, dX.get());
+    IRLBA_HIP_CHECK(hipMemcpyAsync(dX.get() + nr * k, dU.get(), sizeof(double) * nr * k, hipMemcpyDeviceToDevice, s));
+    IRLBA_HIP_CHECK(hipMemcpyAsync(dY.get(), dV.get(), sizeof(double) * nc * k, hipMemcpyDeviceToDevice, s));
+    double* Zc = dX.get();            // accumulate Z' into X[:, :k]
+    double* Yc = dY.get() + nc * k;   // Y'
+    IRLBA_HIP_CHECK(hipMemsetAsync(Yc, 0, sizeof(double) * nc * k, s));
+    int cg_total = 0;
+    if (complement >= 1) {
+        // P g_i = g_i - U J_i and Q h_i = h_i - V K_i.
+        HostMat Jm(k, k), Km(k, k);
+        std::copy(J.begin(), J.end(), Jm.a.begin());
+        std::copy(K.begin(), K.end(), Km.a.begin());
+        auto PG = dalloc<double>(nr * k), QH = dalloc<double>(nc * k);
+        rotate_into(dU.get(), nr, Jm, PG.get());
+        launch_axpby(PG.get(), dGU.get(), 1.0, PG.get(), -1.0, nr * k, s);
+        rotate_into(dV.get(), nc, Km, QH.get());
+        launch_axpby(QH.get(), dGV.get(), 1.0, QH.get(), -1.0, nc * k, s);
+        if (complement == 1) {
+            for (i64 i = 0; i < k; ++i) {
+                launch_axpby(Zc + i * nr, Zc + i * nr, 1.0, PG.get() + i * nr, 1.0 / hs[i], nr, s);
+                launch_axpby(Yc + i * nc, QH.get() + i * nc, 1.0 / hs[i], nullptr, 0.0, nc, s);
+            }
+        } else {
+            auto z = dalloc<double>(nr), t = dalloc<double>(nc), a = dalloc<double>(std::max(nr, nc));
+            for (i64 i = 0; i < k; ++i) {
+                const double sig = hs[i], sig2 = sig * sig;
+                cg_total += bw.cg(true, sig2, dU.get(), PG.get() + i * nr, z.get(), max_cg, cg_tol);
+                cg_total += bw.cg(false, sig2, dV.get(), QH.get() + i * nc, t.get(), max_cg, cg_tol);
+                // Z'_i = s z + A t ; Y'_i = A' z + s t
+                apply_handle(*m, false, t.get(), a.get(), nullptr, nullptr);
+                launch_axpby(Zc + i * nr, Zc + i * nr, 1.0, z.get(), sig, nr, s);
+                launch_axpby(Zc + i * nr, Zc + i * nr, 1.0, a.get(), 1.0, nr, s);
+                apply_handle(*m, true, z.get(), a.get(), nullptr, nullptr);
+                launch_axpby(Yc + i * nc, a.get(), 1.0, t.get(), sig, nc, s);
+            }
+        }
+    }
+    // Outputs.
+    const auto out_kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    IRLBA_HIP_CHECK(hipMemcpyAsync(X, dX.get(), sizeof(double) * nr * R, out_kind, s));
+    IRLBA_HIP_CHECK(hipMemcpyAsync(Y, dY.get(), sizeof(double) * nc * R, out_kind, s));
+    if (dcenter) {
+        // d/dc of A - 1 c' is -(X Y')' 1 = -Y (X' 1).
+        const std::vector<double> ho(static_cast<size_t>(nr), 1.0);
+        auto ones = dalloc<double>(nr), xs = dalloc<double>(R), dc = dalloc<double>(nc);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(ones.get(), ho.data(), sizeof(double) * nr, hipMemcpyHostToDevice, s));
+        bw.dots(dX.get(), nr, R, ones.get(), xs.get());
+        std::vector<double> hx(R);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(hx.data(), xs.get(), sizeof(double) * R, hipMemcpyDeviceToHost, s));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+        HostMat col(R, 1);
+        for (i64 r = 0; r < R; ++r) col(r, 0) = -hx[r];
+        rotate_into(dY.get(), nc, col, dc.get());
+        IRLBA_HIP_CHECK(hipMemcpyAsync(dcenter, dc.get(), sizeof(double) * nc, out_kind, s));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+    return cg_total;
+}
+
 }  // namespace
 
 // ===================================================================== C ABI
@@ -1555,6 +1760,84 @@ int irlba_hip_generate_sparse(irlba_hip_context* ctx, int64_t nrow, int64_t ncol
         m->ctx = ctx;
         m->base = base;
         *out = m;
+    });
+}
+
+int irlba_hip_svd_backward(const irlba_hip_matrix* m, int64_t k, const double* U, const double* D, const double* V,
+                           const double* grad_D, const double* grad_U, const double* grad_V, int complement,
+                           int32_t max_cg_iterations, double cg_tolerance, double* X, double* Y, double* grad_center,
+                           int32_t* cg_iterations, int device_pointers) {
+    return guarded([&] {
+        require(m && U && D && V && X && Y, "null argument");
+        require(complement >= 0 && complement <= 2, "complement must be 0, 1 or 2");
+        require(max_cg_iterations >= 0 && cg_tolerance >= 0, "invalid CG settings");
+        irlba_hip_context* ctx = m->ctx;
+        DeviceGuard g(ctx->device);
+        const i64 nr = m->base->nr, nc = m->base->nc;
+        require(k > 0 && k <= std::min(nr, nc) && k <= 64, "k must lie in [1, min(rows, cols)] and be at most 64");
+        // Absent gradients are zero.
+        std::vector<double> zk, zr, zc;
+        std::shared_ptr<double> dzr, dzc, dzk;
+        auto zeros = [&](i64 n, std::vector<double>& h, std::shared_ptr<double>& d) -> const double* {
+            if (!device_pointers) {
+                h.assign(static_cast<size_t>(n), 0.0);
+                return h.data();
+            }
+            d = dalloc<double>(n);
+            IRLBA_HIP_CHECK(hipMemsetAsync(d.get(), 0, sizeof(double) * n, ctx->stream));
+            return d.get();
+        };
+        const double* w = grad_D ? grad_D : zeros(k, zk, dzk);
+        const double* gu = grad_U ? grad_U : zeros(nr * k, zr, dzr);
+        const double* gv = grad_V ? grad_V : zeros(nc * k, zc, dzc);
+        const int it = run_svd_backward(m, k, U, D, V, w, gu, gv, complement, max_cg_iterations, cg_tolerance, X, Y,
+                                        grad_center, device_pointers != 0);
+        ctx->sync();
+        if (cg_iterations) *cg_iterations = it;
+    });
+}
+
+int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_t* indices,
+                    const int64_t* pointers, int column_major, int64_t rank, const double* X, const double* Y,
+                    double* out, int device_pointers) {
+    return guarded([&] {
+        require(ctx && pointers && X && Y, "null argument");
+        require(nrow >= 0 && ncol >= 0, "negative dimension");
+        require(rank >= 1 && rank <= 128, "rank must lie in [1, 128]");
+        DeviceGuard g(ctx->device);
+        hipStream_t s = ctx->stream;
+        const i64 n_lines = column_major ? ncol : nrow, n_other = column_major ? nrow : ncol;
+        i64 nnz = 0;
+        IRLBA_HIP_CHECK(hipMemcpy(&nnz, pointers + n_lines, sizeof(i64),
+                                  device_pointers ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+        require(nnz >= 0, "pointers are not a valid compressed layout");
+        require(nnz == 0 || (indices && out), "null argument");
+        const auto in_kind = device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        auto dptr = dalloc<i64>(n_lines + 1);
+        auto didx = dalloc<std::int32_t>(nnz);
+        auto dX = dalloc<double>(nrow * rank), dY = dalloc<double>(ncol * rank);
+        auto drm = dalloc<double>(n_other * rank), dout = dalloc<double>(nnz);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(dptr.get(), pointers, sizeof(i64) * (n_lines + 1), in_kind, s));
+        if (nnz) IRLBA_HIP_CHECK(hipMemcpyAsync(didx.get(), indices, sizeof(std::int32_t) * nnz, in_kind, s));
+        IRLBA_HIP_CHECK(hipMemcpyAsync(dX.get(), X, sizeof(double) * nrow * rank, in_kind, s));
+        IRLBA_HIP_CHECK(hipMemcpyAsync(dY.get(), Y, sizeof(double) * ncol * rank, in_kind, s));
+        // The same checks as matrix creation: monotone pointers, indices in range.
+        auto flag = dalloc<int>(1);
+        IRLBA_HIP_CHECK(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
+        launch_validate_compressed(didx.get(), dptr.get(), n_lines, n_other, nnz, flag.get(), s);
+        int hflag = 0;
+        IRLBA_HIP_CHECK(hipMemcpyAsync(&hflag, flag.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+        require(!(hflag & 1), "pointers are not a valid compressed layout");
+        require(!(hflag & 2), "indices out of range");
+        const double* line_f = column_major ? dY.get() : dX.get();
+        const double* entry_f = column_major ? dX.get() : dY.get();
+        launch_to_row_major(entry_f, n_other, n_other, static_cast<int>(rank), drm.get(), s);
+        launch_sddmm(didx.get(), dptr.get(), n_lines, line_f, drm.get(), static_cast<int>(rank), dout.get(), s);
+        if (nnz)
+            IRLBA_HIP_CHECK(hipMemcpyAsync(out, dout.get(), sizeof(double) * nnz,
+                                           device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
     });
 }
 

@@ -1,0 +1,171 @@
+"""GPU parity of the top-k SVD backward (irlba_hip_svd_backward, irlba_hip_sddmm)
+against the oracle's dense restatement and against finite differences.
+
+Tolerances (FP64 throughout):
+  * vs the oracle: max |X Y' - X_o Y_o'| <= 1e-9 of max |X_o Y_o'| (the engine's
+    complement comes from CG solves to 1e-13, the oracle's from a full SVD);
+  * vs central finite differences: 1e-6 of the gradient's max entry;
+  * SDDMM: 1e-13 relative (a dot product of length 2k per entry).
+"""
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import svd_backward as OB
+
+pytestmark = pytest.mark.gpu
+
+irlba = pytest.importorskip("cppirlba_amd")
+
+
+def _triplets(A, k):
+    Uf, sf, Vtf = np.linalg.svd(A, full_matrices=False)
+    return Uf[:, :k].copy(), sf[:k].copy(), Vtf[:k].T.copy()
+
+
+def _grads(m, n, k, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(k), rng.standard_normal((m, k)), rng.standard_normal((n, k))
+
+
+def _dense_of(vals, idx, ptr, nr, nc):
+    A = np.zeros((nr, nc))
+    for c in range(nc):
+        A[idx[ptr[c]:ptr[c + 1]], c] = vals[ptr[c]:ptr[c + 1]]
+    return A
+
+
+def _close(G, ref, tol):
+    return np.abs(G - ref).max() <= tol * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("m,n", [(64, 32), (32, 64)])
+@pytest.mark.parametrize("complement", ["none", "rank_k", "exact"])
+def test_dense_backward_vs_oracle(m, n, complement):
+    k = 5
+    A = np.random.default_rng(1).standard_normal((m, n))
+    U, s, V = _triplets(A, k)
+    w, GU, GV = _grads(m, n, k, 2)
+    h = irlba.HipDenseMatrix(A, storage="f64")
+    res = irlba.svd_backward(h, U, s, V, w, GU, GV, complement=complement)
+    Xo, Yo = OB.svd_backward(A, U, s, V, w, GU, GV, complement)
+    assert res.X.shape == (m, 2 * k) and res.Y.shape == (n, 2 * k)
+    assert _close(res.dense(), Xo @ Yo.T, 1e-9)
+    if complement != "exact":
+        assert res.cg_iterations == 0
+
+
+def test_dense_backward_vs_finite_differences():
+    # 64 x 32 FP64, the whole gradient by central differences of a sign-fixed SVD.
+    m, n, k = 64, 32, 5
+    A = np.random.default_rng(3).standard_normal((m, n))
+    U, s, V = _triplets(A, k)
+    w, GU, GV = _grads(m, n, k, 4)
+    res = irlba.svd_backward(irlba.HipDenseMatrix(A), U, s, V, w, GU, GV, complement="exact")
+    fd = OB.fd_gradient(A, k, w, GU, GV)
+    assert _close(res.dense(), fd, 1e-6)
+    assert res.cg_iterations > 0
+
+
+def test_rank_k_modes_agree_on_rank_k_matrix():
+    rng = np.random.default_rng(5)
+    k = 4
+    A = rng.standard_normal((80, k)) @ rng.standard_normal((k, 60))
+    U, s, V = _triplets(A, k)
+    w, GU, GV = _grads(80, 60, k, 6)
+    h = irlba.HipDenseMatrix(A)
+    r1 = irlba.svd_backward(h, U, s, V, w, GU, GV, complement="rank_k")
+    r2 = irlba.svd_backward(h, U, s, V, w, GU, GV, complement="exact")
+    assert _close(r1.dense(), r2.dense(), 1e-8)
+    Xo, Yo = OB.svd_backward(A, U, s, V, w, GU, GV, "rank_k")
+    assert _close(r1.dense(), Xo @ Yo.T, 1e-10)
+
+
+@pytest.mark.parametrize("nr,nc,column_major", [(500, 300, True), (300, 500, True), (400, 250, False)])
+def test_sparse_backward_and_sddmm(nr, nc, column_major):
+    k = 6
+    if column_major:
+        vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.05, 11, f32=False)
+        A = _dense_of(vals, idx, ptr, nr, nc)
+    else:
+        vals, idx, ptr = O.gen_nb_sparse(nc, nr, 0.05, 11, f32=False)  # CSC of A' = CSR of A
+        A = _dense_of(vals, idx, ptr, nc, nr).T.copy()
+    h = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, column_major=column_major, storage="f64")
+    U, s, V = _triplets(A, k)
+    w, GU, GV = _grads(nr, nc, k, 12)
+    res = irlba.svd_backward(h, U, s, V, w, GU, GV, complement="exact")
+    Xo, Yo = OB.svd_backward(A, U, s, V, w, GU, GV, "exact")
+    G = Xo @ Yo.T
+    assert _close(res.dense(), G, 1e-9)
+    # Gradient of the stored values: the dense gradient on the pattern.
+    gvals = irlba.sddmm(nr, nc, idx, ptr, res.X, res.Y, column_major=column_major)
+    lines = nc if column_major else nr
+    ref = np.empty(idx.size)
+    for l in range(lines):
+        sl = slice(ptr[l], ptr[l + 1])
+        ref[sl] = G[idx[sl], l] if column_major else G[l, idx[sl]]
+    assert np.abs(gvals - ref).max() <= 1e-9 * np.abs(ref).max()
+    direct = np.einsum("ij,ij->i", res.X[(idx if column_major else np.repeat(np.arange(lines), np.diff(ptr)))],
+                       res.Y[(np.repeat(np.arange(lines), np.diff(ptr)) if column_major else idx)])
+    assert np.abs(gvals - direct).max() <= 1e-13 * np.abs(direct).max()
+
+
+def test_sddmm_rank_bounds_and_bad_pattern():
+    X = np.ones((4, 2))
+    Y = np.ones((3, 2))
+    idx = np.array([0, 5], dtype=np.int32)  # row 5 is out of range for 4 rows
+    ptr = np.array([0, 1, 2, 2], dtype=np.int64)
+    with pytest.raises(irlba.HipError, match="out of range"):
+        irlba.sddmm(4, 3, idx, ptr, X, Y)
+    with pytest.raises(irlba.HipError, match="rank"):
+        irlba.sddmm(4, 3, np.array([0, 1], dtype=np.int32), ptr, np.ones((4, 129)), np.ones((3, 129)))
+    out = irlba.sddmm(4, 3, np.zeros(0, dtype=np.int32), np.zeros(4, dtype=np.int64), X, Y)
+    assert out.size == 0
+
+
+def test_centered_backward_and_center_gradient():
+    nr, nc, k = 300, 120, 5
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.1, 21, f32=False)
+    Bm = _dense_of(vals, idx, ptr, nr, nc)
+    c = Bm.mean(axis=0)
+    A = Bm - c[None, :]
+    base = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, storage="f64")
+    h = irlba.CenteredMatrix(base, c)
+    U, s, V = _triplets(A, k)
+    w, GU, GV = _grads(nr, nc, k, 22)
+    res = irlba.svd_backward(h, U, s, V, w, GU, GV, complement="exact", grad_center=True)
+    Xo, Yo = OB.svd_backward(A, U, s, V, w, GU, GV, "exact")
+    assert _close(res.dense(), Xo @ Yo.T, 1e-9)
+    gc = OB.grad_center(Xo, Yo)
+    assert np.abs(res.grad_center - gc).max() <= 1e-9 * max(1.0, np.abs(gc).max())
+
+
+def test_backward_of_gpu_compute():
+    # End to end: triplets from the device solver, then the backward on them.
+    nr, nc, k = 600, 400, 8
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.08, 31, f32=False)
+    A = _dense_of(vals, idx, ptr, nr, nc)
+    h = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, storage="f64")
+    r = irlba.compute(h, k, irlba.Options(convergence_tolerance=1e-12, extra_work=20, exact_for_small_matrix=False))
+    assert r.converged
+    w, GU, GV = _grads(nr, nc, k, 32)
+    res = irlba.svd_backward(h, r.U, r.D, r.V, w, GU, GV, complement="exact")
+    Xo, Yo = OB.svd_backward(A, r.U, r.D, r.V, w, GU, GV, "exact")
+    assert _close(res.dense(), Xo @ Yo.T, 1e-6)
+
+
+def test_backward_argument_errors():
+    A = np.random.default_rng(0).standard_normal((20, 10))
+    U, s, V = _triplets(A, 3)
+    h = irlba.HipDenseMatrix(A)
+    with pytest.raises(irlba.HipError, match="non-increasing"):
+        irlba.svd_backward(h, U, s[::-1].copy(), V)
+    with pytest.raises(ValueError):
+        irlba.svd_backward(h, U[:, :2], s, V)
+    with pytest.raises(KeyError):
+        irlba.svd_backward(h, U, s, V, complement="bogus")
+    # Values-only gradient: U diag(w) V'.
+    w = np.array([1.0, -2.0, 0.5])
+    res = irlba.svd_backward(h, U, s, V, grad_D=w, complement="none")
+    np.testing.assert_allclose(res.dense(), U @ np.diag(w) @ V.T, atol=1e-13)
