@@ -539,3 +539,46 @@ def sddmm(nrow, ncol, indices, pointers, X, Y, column_major=True, ctx=None):
     check(lib().irlba_hip_sddmm(ctx.handle, nrow, ncol, idx.ctypes.data, ptr.ctypes.data, int(column_major),
                                 X.shape[1], X.ctypes.data, Y.ctypes.data, out.ctypes.data, 0))
     return out
+
+
+# ---------------------------------------------------------------- libtorch autograd binding
+_torch_ops_loaded = False
+
+
+def _torch_ops():
+    """torch.ops.irlba_hip (lib/libirlba_torch.so: a torch::autograd::Function in C++)."""
+    global _torch_ops_loaded
+    import os
+
+    import torch
+
+    if not _torch_ops_loaded:
+        lib()  # the engine first, bound to torch's HIP runtime
+        path = os.path.join(os.path.dirname(_lib.LIB_PATH), "libirlba_torch.so")
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is missing: build it with __graft_entry__.build()")
+        torch.ops.load_library(path)
+        _torch_ops_loaded = True
+    return torch.ops.irlba_hip
+
+
+def _op_args(options, storage, complement):
+    o = options or Options()
+    return (int(o.extra_work), float(o.convergence_tolerance), int(o.max_iterations), int(o.seed),
+            _type_code(storage), _COMPLEMENT[complement])
+
+
+def svd_sparse(values, indices, pointers, nrow, ncol, number, center=None, column_major=True, options=None,
+               storage="f64", complement="exact"):
+    """Differentiable top-``number`` SVD of a compressed sparse matrix (torch tensors).
+
+    Returns (U, S, V) in float64; gradients flow to ``values`` (on the stored
+    entries) and to ``center`` (the matrix is A - 1 center' when given).
+    """
+    return _torch_ops().svd_sparse(values, indices, pointers, int(nrow), int(ncol), bool(column_major), center,
+                                   int(number), *_op_args(options, storage, complement))
+
+
+def svd_dense(matrix, number, center=None, options=None, storage="f64", complement="exact"):
+    """Differentiable top-``number`` SVD of a dense torch matrix; gradients to ``matrix`` and ``center``."""
+    return _torch_ops().svd_dense(matrix, center, int(number), *_op_args(options, storage, complement))

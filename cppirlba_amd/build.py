@@ -2,7 +2,9 @@
 
 hipcc compiles every source of csrc/ to an object in _build/ (in parallel)
 and links one shared library.  All HIP code is compiled with packed FP32
-instructions disabled (-target-feature -packed-fp32-ops).
+instructions disabled (-target-feature -packed-fp32-ops).  The libtorch
+autograd binding (csrc/torch_svd.cpp, host code only) is compiled with g++
+against torch's headers into lib/libirlba_torch.so, linked to the engine.
 """
 
 import concurrent.futures
@@ -16,6 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(os.path.dirname(HERE), "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libirlba_hip.so")
+TORCH_LIB = os.path.join(LIBDIR, "libirlba_torch.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 ARCH = os.environ.get("IRLBA_HIP_ARCH", "gfx950")
 
 SOURCES = [
@@ -72,7 +76,30 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_torch()
     return LIB
+
+
+def build_torch():
+    """g++ the libtorch autograd binding against torch's headers and the engine."""
+    import torch
+    from torch.utils import cpp_extension
+
+    tlib = os.path.join(os.path.dirname(torch.__file__), "lib")
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
+           "-Wall", "-Wno-unused-parameter",
+           "-I", INCLUDE, "-I", os.path.join(HERE, "include"),
+           *[f"-I{p}" for p in cpp_extension.include_paths()],
+           os.path.join(CSRC, "torch_svd.cpp"),
+           "-o", TORCH_LIB + ".tmp",
+           "-L", LIBDIR, "-lirlba_hip", "-Wl,-rpath,$ORIGIN",
+           "-L", tlib, "-ltorch", "-ltorch_cpu", "-lc10", f"-Wl,-rpath,{tlib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed on torch_svd.cpp:\n{r.stderr}")
+    os.replace(TORCH_LIB + ".tmp", TORCH_LIB)
+    return TORCH_LIB
 
 
 if __name__ == "__main__":

@@ -1,0 +1,304 @@
+// libtorch binding of the differentiable top-k SVD: a torch::autograd::Function
+// over the engine's C ABI (irlba_hip.h), registered as the operators
+// torch.ops.irlba_hip.svd_sparse / svd_dense.  Host code only; every
+// computation runs in libirlba_hip.so.
+#include <ATen/ATen.h>
+#include <torch/csrc/autograd/custom_function.h>
+#include <torch/cuda.h>
+#include <torch/custom_class.h>
+#include <torch/library.h>
+
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+#include "irlba/torch.hpp"
+#include "irlba_hip.h"
+
+namespace irlba {
+namespace autograd {
+namespace {
+
+using torch::autograd::AutogradContext;
+using torch::autograd::variable_list;
+
+void check(int rc) {
+    if (rc != IRLBA_HIP_OK) throw std::runtime_error(irlba_hip_last_error());
+}
+
+// One engine context per GPU, for the life of the process.
+irlba_hip_context* context_for(int device) {
+    static std::mutex mu;
+    static std::map<int, irlba_hip_context*> ctxs;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = ctxs.find(device);
+    if (it != ctxs.end()) return it->second;
+    irlba_hip_context* c = nullptr;
+    check(irlba_hip_context_create(device, &c));
+    ctxs[device] = c;
+    return c;
+}
+
+// The device handles of one forward, kept for its backward.
+struct Handles : torch::CustomClassHolder {
+    irlba_hip_context* ctx = nullptr;
+    irlba_hip_matrix* base = nullptr;
+    irlba_hip_matrix* op = nullptr;  // base, or the centred view of it
+    ~Handles() override {
+        if (op && op != base) irlba_hip_matrix_destroy(op);
+        if (base) irlba_hip_matrix_destroy(base);
+    }
+};
+
+int value_type(const at::Tensor& t) {
+    switch (t.scalar_type()) {
+        case at::kDouble: return IRLBA_HIP_F64;
+        case at::kFloat: return IRLBA_HIP_F32;
+        case at::kBFloat16: return IRLBA_HIP_BF16;
+        default: throw std::runtime_error("matrix values must be float64, float32 or bfloat16");
+    }
+}
+
+// Device index of the inputs (-1 = host); all must agree.  Device inputs are
+// produced on torch's stream and the engine runs on its own, so wait for torch.
+int common_device(std::initializer_list<const at::Tensor*> ts) {
+    int dev = -2;
+    for (const at::Tensor* t : ts) {
+        if (!t || !t->defined()) continue;
+        const int d = t->is_cuda() ? t->get_device() : -1;
+        if (dev != -2 && d != dev) throw std::runtime_error("all inputs must be on the same device");
+        dev = d;
+    }
+    if (dev >= 0) torch::cuda::synchronize(dev);
+    return dev < 0 ? -1 : dev;
+}
+
+irlba_hip_options make_options(const SvdOptions& o) {
+    irlba_hip_options opt;
+    irlba_hip_options_default(&opt);
+    opt.extra_work = static_cast<int32_t>(o.extra_work);
+    opt.convergence_tolerance = o.convergence_tolerance;
+    opt.max_iterations = static_cast<int32_t>(o.max_iterations);
+    opt.seed = static_cast<uint64_t>(o.seed);
+    return opt;
+}
+
+at::TensorOptions f64_on(int dev) {
+    auto t = at::TensorOptions().dtype(at::kDouble);
+    return dev >= 0 ? t.device(at::Device(at::kCUDA, static_cast<c10::DeviceIndex>(dev))) : t;
+}
+
+// Column-major (rows x k) storage of a (rows x k) gradient, FP64 on `dev`.
+at::Tensor col_major(const at::Tensor& g, int dev) {
+    return g.to(f64_on(dev).device(), at::kDouble).t().contiguous();
+}
+
+// Shared forward: decorate, solve, save.  U, V are returned as transposed views
+// of column-major (k x rows) buffers.
+variable_list solve(AutogradContext* ctx, c10::intrusive_ptr<Handles> h, const std::optional<at::Tensor>& center,
+                    int64_t k, const SvdOptions& o, int dev) {
+    int64_t nr = 0, nc = 0;
+    check(irlba_hip_matrix_dims(h->base, &nr, &nc));
+    h->op = h->base;
+    at::Tensor c;
+    if (center.has_value() && center->defined()) {
+        TORCH_CHECK(center->dim() == 1 && center->size(0) == nc, "center must have one entry per column");
+        c = center->detach().to(f64_on(dev).device(), at::kDouble).contiguous();
+        check(irlba_hip_centered_create(h->base, c.data_ptr<double>(), dev >= 0, &h->op));
+    }
+    TORCH_CHECK(k >= 1 && k <= std::min(nr, nc), "k must lie in [1, min(rows, cols)]");
+    at::Tensor Ucm = at::empty({k, nr}, f64_on(dev)), Vcm = at::empty({k, nc}, f64_on(dev)),
+               S = at::empty({k}, f64_on(dev));
+    irlba_hip_options opt = make_options(o);
+    int64_t nout = 0;
+    int32_t conv = 0, iters = 0;
+    check(irlba_hip_compute(h->op, k, &opt, Ucm.data_ptr<double>(), Vcm.data_ptr<double>(), S.data_ptr<double>(),
+                            &nout, &conv, &iters, dev >= 0));
+    TORCH_CHECK(nout == k, "the solver returned fewer triplets than requested");
+    ctx->saved_data["handles"] = h;
+    ctx->saved_data["k"] = k;
+    ctx->saved_data["dev"] = static_cast<int64_t>(dev);
+    ctx->saved_data["complement"] = o.complement;
+    ctx->saved_data["center"] = c.defined();
+    ctx->save_for_backward({Ucm, S, Vcm});
+    return {Ucm.t(), S, Vcm.t()};
+}
+
+// Factored dL/dA = X Y' (X: 2k x rows, Y: 2k x cols, both column-major views)
+// and the centre gradient, from the saved state and the incoming gradients.
+// `center_edge` is the centre's position among the tensor inputs (the indices
+// that needs_input_grad() takes).
+struct Factored {
+    at::Tensor X, Y, dcenter;
+};
+
+Factored factored_grad(AutogradContext* ctx, const variable_list& g, size_t center_edge) {
+    auto saved = ctx->get_saved_variables();
+    const at::Tensor &Ucm = saved[0], &S = saved[1], &Vcm = saved[2];
+    auto h = ctx->saved_data["handles"].toCustomClass<Handles>();
+    const int64_t k = ctx->saved_data["k"].toInt();
+    const int dev = static_cast<int>(ctx->saved_data["dev"].toInt());
+    const int complement = static_cast<int>(ctx->saved_data["complement"].toInt());
+    const bool want_center = ctx->saved_data["center"].toBool() && ctx->needs_input_grad(center_edge);
+    const int64_t nr = Ucm.size(1), nc = Vcm.size(1);
+    if (dev >= 0) torch::cuda::synchronize(dev);
+    at::Tensor gU = g[0].defined() ? col_major(g[0], dev) : at::Tensor();
+    at::Tensor gS = g[1].defined() ? g[1].to(f64_on(dev).device(), at::kDouble).contiguous() : at::Tensor();
+    at::Tensor gV = g[2].defined() ? col_major(g[2], dev) : at::Tensor();
+    if (dev >= 0) torch::cuda::synchronize(dev);
+    Factored f;
+    f.X = at::empty({2 * k, nr}, f64_on(dev));
+    f.Y = at::empty({2 * k, nc}, f64_on(dev));
+    if (want_center) f.dcenter = at::empty({nc}, f64_on(dev));
+    auto p = [](const at::Tensor& t) -> double* { return t.defined() ? t.data_ptr<double>() : nullptr; };
+    int32_t cg = 0;
+    check(irlba_hip_svd_backward(h->op, k, p(Ucm), p(S), p(Vcm), p(gS), p(gU), p(gV), complement, 1000, 1e-12,
+                                 p(f.X), p(f.Y), p(f.dcenter), &cg, dev >= 0));
+    return f;
+}
+
+class SparseSvd : public torch::autograd::Function<SparseSvd> {
+public:
+    // Arguments: 0 values, 1 indices, 2 pointers, 3 nrow, 4 ncol, 5 column_major,
+    // 6 center, 7 k, 8 options; tensor inputs: values, indices, pointers, center.
+    static variable_list forward(AutogradContext* ctx, at::Tensor values, at::Tensor indices, at::Tensor pointers,
+                                 int64_t nrow, int64_t ncol, bool column_major, std::optional<at::Tensor> center,
+                                 int64_t k, SvdOptions o) {
+        TORCH_CHECK(indices.scalar_type() == at::kInt, "indices must be int32");
+        TORCH_CHECK(pointers.scalar_type() == at::kLong, "pointers must be int64");
+        TORCH_CHECK(values.scalar_type() == at::kDouble || values.scalar_type() == at::kFloat,
+                    "sparse values must be float64 or float32");
+        const at::Tensor* cptr = center.has_value() ? &*center : nullptr;
+        const int dev = common_device({&values, &indices, &pointers, cptr});
+        at::Tensor x = values.detach().contiguous(), i = indices.contiguous(), pp = pointers.contiguous();
+        auto h = c10::make_intrusive<Handles>();
+        h->ctx = context_for(dev < 0 ? 0 : dev);
+        const int storage = static_cast<int>(o.storage);
+        check(irlba_hip_sparse_create(h->ctx, nrow, ncol, value_type(x), x.data_ptr(), i.data_ptr<int32_t>(),
+                                      pp.data_ptr<int64_t>(), column_major, storage, storage == IRLBA_HIP_F64,
+                                      dev >= 0, &h->base));
+        ctx->saved_data["nrow"] = nrow;
+        ctx->saved_data["ncol"] = ncol;
+        ctx->saved_data["column_major"] = column_major;
+        ctx->saved_data["indices"] = i;
+        ctx->saved_data["pointers"] = pp;
+        ctx->saved_data["values_dtype"] = static_cast<int64_t>(values.scalar_type());
+        return solve(ctx, h, center, k, o, dev);
+    }
+
+    static variable_list backward(AutogradContext* ctx, variable_list g) {
+        Factored f = factored_grad(ctx, g, 3);
+        at::Tensor dvalues;
+        if (ctx->needs_input_grad(0)) {
+            auto h = ctx->saved_data["handles"].toCustomClass<Handles>();
+            const int dev = static_cast<int>(ctx->saved_data["dev"].toInt());
+            at::Tensor i = ctx->saved_data["indices"].toTensor(), pp = ctx->saved_data["pointers"].toTensor();
+            dvalues = at::empty({i.numel()}, f64_on(dev));
+            check(irlba_hip_sddmm(h->ctx, ctx->saved_data["nrow"].toInt(), ctx->saved_data["ncol"].toInt(),
+                                  i.data_ptr<int32_t>(), pp.data_ptr<int64_t>(),
+                                  ctx->saved_data["column_major"].toBool(), f.X.size(0), f.X.data_ptr<double>(),
+                                  f.Y.data_ptr<double>(), dvalues.data_ptr<double>(), dev >= 0));
+            dvalues = dvalues.to(static_cast<at::ScalarType>(ctx->saved_data["values_dtype"].toInt()));
+        }
+        return {dvalues, at::Tensor(), at::Tensor(), at::Tensor(), at::Tensor(), at::Tensor(), f.dcenter,
+                at::Tensor(), at::Tensor()};
+    }
+};
+
+class DenseSvd : public torch::autograd::Function<DenseSvd> {
+public:
+    // Arguments: 0 matrix, 1 center, 2 k, 3 options; tensor inputs: matrix, center.
+    static variable_list forward(AutogradContext* ctx, at::Tensor matrix, std::optional<at::Tensor> center, int64_t k,
+                                 SvdOptions o) {
+        TORCH_CHECK(matrix.dim() == 2, "matrix must be two-dimensional");
+        const at::Tensor* cptr = center.has_value() ? &*center : nullptr;
+        const int dev = common_device({&matrix, cptr});
+        at::Tensor a = matrix.detach();
+        int column_major = 0;
+        if (!a.is_contiguous()) {
+            if (a.t().is_contiguous()) column_major = 1;
+            else a = a.contiguous();
+        }
+        auto h = c10::make_intrusive<Handles>();
+        h->ctx = context_for(dev < 0 ? 0 : dev);
+        check(irlba_hip_dense_create(h->ctx, a.size(0), a.size(1), value_type(a), a.data_ptr(), column_major,
+                                     static_cast<int>(o.storage), dev >= 0, &h->base));
+        ctx->saved_data["matrix_dtype"] = static_cast<int64_t>(matrix.scalar_type());
+        return solve(ctx, h, center, k, o, dev);
+    }
+
+    static variable_list backward(AutogradContext* ctx, variable_list g) {
+        Factored f = factored_grad(ctx, g, 1);
+        at::Tensor dA;
+        if (ctx->needs_input_grad(0))
+            dA = at::mm(f.X.t(), f.Y).to(static_cast<at::ScalarType>(ctx->saved_data["matrix_dtype"].toInt()));
+        return {dA, f.dcenter, at::Tensor(), at::Tensor()};
+    }
+};
+
+SvdOptions options_of(int64_t extra_work, double tol, int64_t max_iterations, int64_t seed, int64_t storage,
+                      int64_t complement) {
+    TORCH_CHECK(complement >= 0 && complement <= 2, "complement must be 0, 1 or 2");
+    SvdOptions o;
+    o.extra_work = extra_work;
+    o.convergence_tolerance = tol;
+    o.max_iterations = max_iterations;
+    o.seed = seed;
+    o.storage = storage;
+    o.complement = complement;
+    return o;
+}
+
+std::tuple<at::Tensor, at::Tensor, at::Tensor> op_sparse(const at::Tensor& values, const at::Tensor& indices,
+                                                          const at::Tensor& pointers, int64_t nrow, int64_t ncol,
+                                                          bool column_major, const std::optional<at::Tensor>& center,
+                                                          int64_t k, int64_t extra_work, double tol,
+                                                          int64_t max_iterations, int64_t seed, int64_t storage,
+                                                          int64_t complement) {
+    return svd_sparse(values, indices, pointers, nrow, ncol, column_major, center, k,
+                      options_of(extra_work, tol, max_iterations, seed, storage, complement));
+}
+
+std::tuple<at::Tensor, at::Tensor, at::Tensor> op_dense(const at::Tensor& matrix,
+                                                         const std::optional<at::Tensor>& center, int64_t k,
+                                                         int64_t extra_work, double tol, int64_t max_iterations,
+                                                         int64_t seed, int64_t storage, int64_t complement) {
+    return svd_dense(matrix, center, k, options_of(extra_work, tol, max_iterations, seed, storage, complement));
+}
+
+}  // namespace
+
+std::tuple<at::Tensor, at::Tensor, at::Tensor> svd_sparse(const at::Tensor& values, const at::Tensor& indices,
+                                                           const at::Tensor& pointers, int64_t nrow, int64_t ncol,
+                                                           bool column_major, const std::optional<at::Tensor>& center,
+                                                           int64_t k, const SvdOptions& options) {
+    auto out = SparseSvd::apply(values, indices, pointers, nrow, ncol, column_major, center, k, options);
+    return {out[0], out[1], out[2]};
+}
+
+std::tuple<at::Tensor, at::Tensor, at::Tensor> svd_dense(const at::Tensor& matrix,
+                                                          const std::optional<at::Tensor>& center, int64_t k,
+                                                          const SvdOptions& options) {
+    auto out = DenseSvd::apply(matrix, center, k, options);
+    return {out[0], out[1], out[2]};
+}
+
+}  // namespace autograd
+}  // namespace irlba
+
+TORCH_LIBRARY(irlba_hip, m) {
+    m.class_<irlba::autograd::Handles>("Handles");
+    m.def(
+        "svd_sparse(Tensor values, Tensor indices, Tensor pointers, int nrow, int ncol, bool column_major, "
+        "Tensor? center, int k, int extra_work, float tol, int max_iterations, int seed, int storage, "
+        "int complement) -> (Tensor, Tensor, Tensor)");
+    m.def(
+        "svd_dense(Tensor matrix, Tensor? center, int k, int extra_work, float tol, int max_iterations, int seed, "
+        "int storage, int complement) -> (Tensor, Tensor, Tensor)");
+}
+
+TORCH_LIBRARY_IMPL(irlba_hip, CompositeImplicitAutograd, m) {
+    m.impl("svd_sparse", &irlba::autograd::op_sparse);
+    m.impl("svd_dense", &irlba::autograd::op_dense);
+}

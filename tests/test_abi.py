@@ -69,3 +69,17 @@ def test_no_device_reports_error_not_crash():
         pass
     with pytest.raises(irlba.HipError):
         irlba.Context(0)
+
+
+def test_torch_binding_registers_operators():
+    """libirlba_torch.so (the torch::autograd::Function binding) loads and
+    registers its operators; without a GPU a call raises instead of falling back."""
+    torch = pytest.importorskip("torch")
+    import cppirlba_amd as irlba
+
+    ops = irlba._torch_ops()
+    assert "svd_sparse(Tensor values" in str(ops.svd_sparse.default._schema)
+    assert "svd_dense(Tensor matrix" in str(ops.svd_dense.default._schema)
+    if not torch.cuda.is_available():
+        with pytest.raises(RuntimeError):
+            irlba.svd_dense(torch.randn(6, 4, dtype=torch.float64, requires_grad=True), 2)
