@@ -107,9 +107,99 @@ def spmv_mode(args):
                       "copy_bytes": n * 8}), flush=True)
 
 
+def _sparse_uniform_csc(torch, m, n, density, gen, dev, chunk=1000):
+    """CSC of an m x n matrix with Bernoulli(density) positions and N(0,1) FP32 values (on the device)."""
+    rows, counts = [], []
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        mask = torch.rand((c1 - c0, m), generator=gen, device=dev) < density  # one row per column
+        nzc, nzr = mask.nonzero(as_tuple=True)  # column-major order: by column, then row
+        rows.append(nzr.to(torch.int32))
+        counts.append(torch.bincount(nzc, minlength=c1 - c0))
+    idx = torch.cat(rows)
+    ptr = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+    ptr[1:] = torch.cumsum(torch.cat(counts), 0)
+    vals = torch.randn(idx.numel(), generator=gen, device=dev, dtype=torch.float32)
+    return vals, idx, ptr
+
+
+def _time_backward(torch, fwd, loss_of, reps):
+    """Forward once, then `reps` timed backward passes through the autograd graph."""
+    out = fwd()
+    L = loss_of(*out)
+    times = []
+    for r in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.backward(retain_graph=True)
+        torch.cuda.synchronize()
+        if r:  # the first pass pays one-time allocations
+            times.append(1e3 * (time.perf_counter() - t0))
+    return out, times
+
+
+def backward_leg(args):
+    """Differentiable top-16 SVD: dense 4096 x 1024 FP32 (Gaussian + planted rank 16,
+    s_i = 400 * 0.95^i) and sparse 50000 x 20000 at 1% (uniform pattern, N(0,1)
+    FP32 values); loss sum w_i s_i + <G_U, U> + <G_V, V> with seeded Gaussian w, G;
+    backward through the torch::autograd::Function binding."""
+    import numpy as np
+    import torch
+
+    import cppirlba_amd as irlba
+
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed)
+    k = 16
+    res = {"number": k, "loss": "sum w_i s_i + <G_U,U> + <G_V,V>, seeded Gaussian w, G"}
+    opts = irlba.Options(extra_work=args.extra_work, convergence_tolerance=1e-8)
+
+    def loss_fn(w, GU, GV):
+        return lambda U, S, V: (w * S).sum() + (GU * U).sum() + (GV * V).sum()
+
+    # Dense.
+    m, n = 4096, 1024
+    Q1, _ = torch.linalg.qr(torch.randn(m, k, generator=gen, device=dev, dtype=torch.float64))
+    Q2, _ = torch.linalg.qr(torch.randn(n, k, generator=gen, device=dev, dtype=torch.float64))
+    s = 400.0 * 0.95 ** torch.arange(k, device=dev, dtype=torch.float64)
+    A = ((Q1 * s) @ Q2.T + torch.randn(m, n, generator=gen, device=dev, dtype=torch.float64)).float()
+    A.requires_grad_(True)
+    w = torch.randn(k, generator=gen, device=dev, dtype=torch.float64)
+    GU = torch.randn(m, k, generator=gen, device=dev, dtype=torch.float64)
+    GV = torch.randn(n, k, generator=gen, device=dev, dtype=torch.float64)
+    for comp in ("exact", "rank_k"):
+        t0 = time.perf_counter()
+        _, times = _time_backward(torch, lambda: irlba.svd_dense(A, k, options=opts, storage="f32",
+                                                                 complement=comp),
+                                  loss_fn(w, GU, GV), args.reps)
+        res[f"dense_{m}x{n}_{comp}_ms"] = round(float(np.median(times)), 3)
+        log(f"  backward dense {m}x{n} ({comp}): {np.median(times):.3f} ms "
+            f"(forward + {args.reps + 1} backward {time.perf_counter() - t0:.2f} s)")
+    del A, GU, GV
+
+    # Sparse.
+    m, n, density = 50000, 20000, 0.01
+    vals, idx, ptr = _sparse_uniform_csc(torch, m, n, density, gen, dev)
+    vals.requires_grad_(True)
+    res["sparse_nnz"] = int(idx.numel())
+    GU = torch.randn(m, k, generator=gen, device=dev, dtype=torch.float64)
+    GV = torch.randn(n, k, generator=gen, device=dev, dtype=torch.float64)
+    comps = ("rank_k", "exact") if args.backward_exact_sparse else ("rank_k",)
+    for comp in comps:
+        t0 = time.perf_counter()
+        _, times = _time_backward(torch, lambda: irlba.svd_sparse(vals, idx, ptr, m, n, k, options=opts,
+                                                                  storage="f32", complement=comp),
+                                  loss_fn(w, GU, GV), args.reps)
+        res[f"sparse_{m}x{n}_{comp}_ms"] = round(float(np.median(times)), 3)
+        log(f"  backward sparse {m}x{n} nnz={idx.numel()} ({comp}): {np.median(times):.3f} ms "
+            f"(forward + {args.reps + 1} backward {time.perf_counter() - t0:.2f} s)")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", choices=["solve", "spmv"], default="solve")
+    ap.add_argument("--mode", choices=["solve", "spmv", "backward"], default="solve")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--storage", default="f32")
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +213,9 @@ def main():
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-backward", action="store_true")
+    ap.add_argument("--backward-exact-sparse", action="store_true",
+                    help="also time the exact (CG) complement on the sparse backward case")
     ap.add_argument("--cpu-sample-cols", type=int, default=33334)
     args = ap.parse_args()
 
@@ -132,6 +225,9 @@ def main():
         sys.exit(2)
     if args.mode == "spmv":
         spmv_mode(args)
+        return
+    if args.mode == "backward":
+        print(json.dumps(backward_leg(args)), flush=True)
         return
 
     import numpy as np
@@ -251,6 +347,10 @@ def main():
         out["sv_rel_err_vs_fp64"] = rel
         out["max_subspace_angle_vs_fp64"] = angle
         del fp64
+
+    if not args.skip_backward:
+        log("SVD backward (top-16, dense and sparse)")
+        out["svd_backward"] = backward_leg(args)
 
     if not args.skip_cpu and nnz:
         log("CPU baseline (oracle restatement of ParallelSparseMatrix) on a column slice")

@@ -524,8 +524,11 @@ def svd_backward(matrix, U, D, V, grad_D=None, grad_U=None, grad_V=None, complem
     return SvdBackward(X, Y, gc, its.value)
 
 
-def sddmm(nrow, ncol, indices, pointers, X, Y, column_major=True, ctx=None):
-    """out[e] = X[row_e] . Y[col_e] over the stored entries of a compressed pattern."""
+def sddmm(nrow, ncol, indices, pointers, X, Y, column_major=True, out_type="f64", ctx=None):
+    """out[e] = X[row_e] . Y[col_e] over the stored entries of a compressed pattern.
+
+    ``out_type`` "f64" or "f32" (FP32 factors, FP64 accumulation, FP32 output).
+    """
     ctx = ctx or default_context()
     X = _colmajor(X)
     Y = _colmajor(Y)
@@ -535,9 +538,10 @@ def sddmm(nrow, ncol, indices, pointers, X, Y, column_major=True, ctx=None):
     ptr = np.ascontiguousarray(pointers, dtype=np.int64)
     if ptr.size != (ncol if column_major else nrow) + 1:
         raise ValueError("pointers have the wrong length")
-    out = np.zeros(idx.size)
+    code = _type_code(out_type)
+    out = np.zeros(idx.size, dtype=np.float32 if code == F32 else np.float64)
     check(lib().irlba_hip_sddmm(ctx.handle, nrow, ncol, idx.ctypes.data, ptr.ctypes.data, int(column_major),
-                                X.shape[1], X.ctypes.data, Y.ctypes.data, out.ctypes.data, 0))
+                                X.shape[1], X.ctypes.data, Y.ctypes.data, out.ctypes.data, code, 0))
     return out
 
 

@@ -201,10 +201,24 @@ void launch_axpy_dev(double* y, const double* src, const double* coef, i64 n, hi
 void launch_scale_columns(double* M, i64 ld, i64 L, int k, const double* d, hipStream_t s);
 
 // Differentiable SVD (kernels_backward.hip).
-void launch_to_row_major(const double* X, i64 rows, i64 ld, int R, double* out, hipStream_t s);
-void launch_sddmm(const std::int32_t* idx, const i64* ptr, i64 n_lines, const double* line_f, const double* entry_f_rm,
-                  int R, double* out, hipStream_t s);
+// Row stride of the packed SDDMM factors for rank R (FP32 or FP64 factors).
+int sddmm_padded_rank(int R, bool f32, int* lanes_per_entry = nullptr);
+void launch_pack_rows(const double* X, i64 rows, i64 ld, int R, int Rp, bool f32, void* out, hipStream_t s);
+void launch_sddmm(const std::int32_t* idx, const i64* ptr, i64 n_lines, const void* line_rm, const void* entry_rm,
+                  int R, bool f32, void* out, hipStream_t s);
+int gram_blocks(i64 L);
+// out[i * ld_out + j] = A[:, i] . B[:, j] for L x k operands (k <= 64); partials
+// holds gram_blocks(L) * k * k doubles.
+void launch_gram(const double* A, i64 lda, const double* B, i64 ldb, int k, i64 L, double* partials, double* out,
+                 int ld_out, hipStream_t s);
+// out[i] = A[:, i] . B[:, i]; partials holds gram_blocks(L) * k doubles.
+void launch_coldots(const double* A, i64 lda, const double* B, i64 ldb, int k, i64 L, double* partials, double* out,
+                    hipStream_t s);
+void launch_cols_axpby(double* out, i64 ldo, const double* a_coef, double a_scale, const double* x, i64 ldx,
+                       const double* b_coef, double b_scale, int k, i64 L, hipStream_t s);
+void launch_cg_coefs(const double* a, double* b, double* out, int k, int mode, hipStream_t s);
 void launch_axpby(double* out, const double* a, double alpha, const double* b, double beta, i64 n, hipStream_t s);
+void launch_fill(double* out, double v, i64 n, hipStream_t s);
 
 // Synthetic data (counter-based, identical to the oracle's generator).
 void launch_gen_nb_count(i64 n_minor, i64 n_major, std::uint32_t thr, std::uint64_t seed, i64* counts,

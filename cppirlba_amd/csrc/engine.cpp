@@ -124,7 +124,39 @@ struct irlba_hip_context {
         IRLBA_HIP_CHECK(hipStreamSynchronize(stream));
         resolve_events();
     }
+    // Grow-only device scratch for the temporaries of one call.  Calls on a
+    // context are serialised on its stream and each ends with a sync, so one
+    // buffer serves them all.
+    std::shared_ptr<void> ws;
+    size_t ws_bytes = 0;
+    char* workspace(size_t bytes) {
+        if (bytes > ws_bytes) {
+            IRLBA_HIP_CHECK(hipStreamSynchronize(stream));
+            ws.reset();
+            ws = dalloc_bytes(bytes);
+            ws_bytes = bytes;
+        }
+        return static_cast<char*>(ws.get());
+    }
 };
+
+namespace {
+
+// Carves 256-byte aligned arrays out of one buffer; size() on a null base
+// measures what the carving needs.
+struct Carve {
+    char* base = nullptr;
+    size_t off = 0;
+    template <typename T>
+    T* take(i64 n) {
+        off = (off + 255) / 256 * 256;
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += sizeof(T) * static_cast<size_t>(std::max<i64>(n, 1));
+        return p;
+    }
+};
+
+}  // namespace
 
 namespace {
 
@@ -1088,7 +1120,7 @@ irlba_hip_matrix* decorate_scale(const irlba_hip_matrix* in, const std::shared_p
 
 // ------------------------------------------------------------------ differentiable top-k SVD
 // dL/dA for L = sum_i w_i s_i + <G_U, U> + <G_V, V> as X Y' with
-//   X = [U M + Z' | U],  Y = [V | Y'],
+//   X = [U | U M + Z'],  Y = [Y' | V],
 //   M = diag(w) + (F o (J - J')) S + S (F o (K - K')),  J = U'G_U, K = V'G_V,
 //   F_ij = 1 / (s_j^2 - s_i^2) (i != j).
 // Complement columns (the part of dU, dV outside the top-k subspaces):
@@ -1097,75 +1129,68 @@ irlba_hip_matrix* decorate_scale(const irlba_hip_matrix* in, const std::shared_p
 //           (s_i^2 I - P A A' P) z_i = P g_i on P = I - U U' and
 //           (s_i^2 I - Q A'A Q) t_i = Q h_i on Q = I - V V', solved by CG
 //           (positive definite when s_k > s_{k+1}).
-// Projections are applied as y - U (U'y) with the Gram-Schmidt kernels; no
-// dense I - UU' is formed.
-struct Backward {
-    irlba_hip_context* ctx;
+// Projections are applied as y - U (U'y) (a Gram product and a rotation); no
+// dense I - UU' is formed.  The 2k CG systems run in lock step: one iteration
+// is 2k products of each kind plus column-batched kernels, with one host
+// synchronisation for the convergence test.
+
+// One set of k shifted systems (sig2_i I - P B P) x_i = b_i on the complement
+// of `basis` (L x k), B = A A' (left, L = rows) or A' A (right, L = cols).
+struct CgSet {
+    bool left = true;
+    i64 L = 0, Lo = 0;
+    const double* basis = nullptr;
+    double *b = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *uc = nullptr, *tmp = nullptr;
+    double *proj = nullptr;                                       // k x ldy Gram output (rotation layout)
+    double *rr = nullptr, *rr_new = nullptr, *pap = nullptr, *alpha = nullptr, *beta = nullptr;  // k each
+    double* sig2 = nullptr;                                       // k
+    double* partials = nullptr;                                   // Gram / column-dot partials
+};
+
+struct LockstepCg {
     const irlba_hip_matrix* m;
     hipStream_t s;
-    i64 nr, nc, k;
-    std::shared_ptr<double> partials, tvec, norm_part, scal;
+    int k, ldy;
 
-    Backward(irlba_hip_context* c, const irlba_hip_matrix* mm, i64 k_)
-        : ctx(c), m(mm), s(c->stream), nr(mm->base->nr), nc(mm->base->nc), k(k_) {
-        const int G = std::max(cgs_blocks(nr), cgs_blocks(nc));
-        partials = dalloc<double>(static_cast<i64>(G) * std::max<i64>(2 * k, 1));
-        tvec = dalloc<double>(std::max<i64>(2 * k, 1));
-        norm_part = dalloc<double>(G);
-        scal = dalloc<double>(4);
-    }
-
-    // out[0..n) = M[:, :n]' f on the device.
-    void dots(const double* M, i64 L, i64 n, const double* f, double* out) {
-        launch_cgs_dots(M, L, static_cast<int>(n), f, L, partials.get(), s);
-        launch_cgs_reduce(partials.get(), L, static_cast<int>(n), out, s);
-    }
-    double dot(const double* a, const double* b, i64 L) {
-        dots(a, L, 1, b, scal.get());
-        double h = 0;
-        IRLBA_HIP_CHECK(hipMemcpyAsync(&h, scal.get(), sizeof(double), hipMemcpyDeviceToHost, s));
-        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
-        return h;
-    }
-    // y -= M[:, :n] (M[:, :n]' y)
-    void project(const double* M, i64 L, i64 n, double* y) {
-        launch_cgs_dots(M, L, static_cast<int>(n), y, L, partials.get(), s);
-        launch_cgs_reduce(partials.get(), L, static_cast<int>(n), tvec.get(), s);
-        launch_cgs_update(M, L, static_cast<int>(n), tvec.get(), y, L, norm_part.get(), s);
-    }
-    // CG for (sig2 I - P B P) x = b on the complement of `basis` (L x k, P =
-    // I - basis basis'), with B = A A' (left, L = rows) or A' A (right, L = cols);
-    // b lies in the complement and so do all iterates.
-    int cg(bool left, double sig2, const double* basis, const double* b, double* x, int max_it, double tol) {
-        const i64 L = left ? nr : nc, Lo = left ? nc : nr;
-        auto r = dalloc<double>(L), p = dalloc<double>(L), ap = dalloc<double>(L), tmp = dalloc<double>(Lo);
-        auto op = [&](const double* in, double* out) {
-            apply_handle(*m, left, in, tmp.get(), nullptr, nullptr);    // A' in (left) or A in (right)
-            apply_handle(*m, !left, tmp.get(), out, nullptr, nullptr);  // A A' in or A' A in
-            project(basis, L, k, out);
-            launch_axpby(out, in, sig2, out, -1.0, L, s);               // sig2 in - P B in
-        };
-        launch_axpby(x, b, 1.0 / sig2, nullptr, 0.0, L, s);  // start from the rank-k answer
-        op(x, ap.get());
-        launch_axpby(r.get(), b, 1.0, ap.get(), -1.0, L, s);
-        IRLBA_HIP_CHECK(hipMemcpyAsync(p.get(), r.get(), sizeof(double) * L, hipMemcpyDeviceToDevice, s));
-        const double bnorm = std::sqrt(dot(b, b, L));
-        double rr = dot(r.get(), r.get(), L);
-        int it = 0;
-        while (it < max_it && std::sqrt(rr) > tol * bnorm) {
-            op(p.get(), ap.get());
-            const double pap = dot(p.get(), ap.get(), L);
-            if (!(pap > 0)) throw std::runtime_error("complement system is not positive definite (needs s_k > s_{k+1})");
-            const double alpha = rr / pap;
-            launch_axpby(x, x, 1.0, p.get(), alpha, L, s);
-            launch_axpby(r.get(), r.get(), 1.0, ap.get(), -alpha, L, s);
-            const double rr_new = dot(r.get(), r.get(), L);
-            launch_axpby(p.get(), r.get(), 1.0, p.get(), rr_new / rr, L, s);
-            rr = rr_new;
-            ++it;
+    // out = sig2 p - P B p for every column, into set.ap.
+    void op(CgSet& c, const double* in) {
+        for (int i = 0; i < k; ++i) {
+            apply_handle(*m, c.left, in + static_cast<i64>(i) * c.L, c.tmp + static_cast<i64>(i) * c.Lo, nullptr,
+                         nullptr);
+            apply_handle(*m, !c.left, c.tmp + static_cast<i64>(i) * c.Lo, c.ap + static_cast<i64>(i) * c.L, nullptr,
+                         nullptr);
         }
-        project(basis, L, k, x);  // drop the rounding drift back into span(basis)
-        return it;
+        // ap = basis (basis' ap) - ap + sig2 in
+        launch_gram(c.basis, c.L, c.ap, c.L, k, c.L, c.partials, c.proj, ldy, s);
+        launch_rotate(c.basis, c.L, k, c.proj, ldy, k, c.uc, c.L, c.L, s);
+        launch_cols_axpby(c.ap, c.L, nullptr, -1.0, c.uc, c.L, nullptr, 1.0, k, c.L, s);
+        launch_cols_axpby(c.ap, c.L, nullptr, 1.0, in, c.L, c.sig2, 1.0, k, c.L, s);
+    }
+    void start(CgSet& c) {
+        // x0 = b / sig2 (the rank-k answer), r0 = b - op(x0), p0 = r0.
+        launch_cg_coefs(c.sig2, nullptr, c.alpha, k, 3, s);  // alpha = 1 / sig2
+        launch_cols_axpby(c.x, c.L, nullptr, 0.0, c.b, c.L, c.alpha, 1.0, k, c.L, s);
+        op(c, c.x);
+        launch_cols_axpby(c.r, c.L, nullptr, 0.0, c.b, c.L, nullptr, 1.0, k, c.L, s);
+        launch_cols_axpby(c.r, c.L, nullptr, 1.0, c.ap, c.L, nullptr, -1.0, k, c.L, s);
+        launch_cols_axpby(c.p, c.L, nullptr, 0.0, c.r, c.L, nullptr, 1.0, k, c.L, s);
+        launch_coldots(c.r, c.L, c.r, c.L, k, c.L, c.partials, c.rr, s);
+    }
+    void step(CgSet& c) {
+        op(c, c.p);
+        launch_coldots(c.p, c.L, c.ap, c.L, k, c.L, c.partials, c.pap, s);
+        launch_cg_coefs(c.rr, c.pap, c.alpha, k, 0, s);  // alpha = rr / pap (0 where pap <= 0)
+        launch_cols_axpby(c.x, c.L, nullptr, 1.0, c.p, c.L, c.alpha, 1.0, k, c.L, s);
+        launch_cols_axpby(c.r, c.L, nullptr, 1.0, c.ap, c.L, c.alpha, -1.0, k, c.L, s);
+        launch_coldots(c.r, c.L, c.r, c.L, k, c.L, c.partials, c.rr_new, s);
+        launch_cg_coefs(c.rr_new, c.rr, c.beta, k, 1, s);  // beta = rr_new / rr; rr = rr_new
+        launch_cols_axpby(c.p, c.L, c.beta, 1.0, c.r, c.L, nullptr, 1.0, k, c.L, s);
+    }
+    void finish(CgSet& c) {
+        // Drop the rounding drift of x back out of span(basis).
+        launch_gram(c.basis, c.L, c.x, c.L, k, c.L, c.partials, c.proj, ldy, s);
+        launch_rotate(c.basis, c.L, k, c.proj, ldy, k, c.uc, c.L, c.L, s);
+        launch_cols_axpby(c.x, c.L, nullptr, 1.0, c.uc, c.L, nullptr, -1.0, k, c.L, s);
     }
 };
 
@@ -1174,16 +1199,62 @@ int run_svd_backward(const irlba_hip_matrix* m, i64 k, const double* U, const do
                      double* X, double* Y, double* dcenter, bool device_ptrs) {
     irlba_hip_context* ctx = m->ctx;
     hipStream_t s = ctx->stream;
-    Backward bw(ctx, m, k);
-    const i64 nr = bw.nr, nc = bw.nc, R = 2 * k;
+    const i64 nr = m->base->nr, nc = m->base->nc, R = 2 * k;
     require(k > 0 && k <= std::min(nr, nc) && k <= 64, "k must lie in [1, min(rows, cols)] and be at most 64");
-    const auto in_kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    auto up = [&](const double* src, i64 n) {
-        auto d = dalloc<double>(n);
-        IRLBA_HIP_CHECK(hipMemcpyAsync(d.get(), src, sizeof(double) * n, in_kind, s));
-        return d;
+    const int ki = static_cast<int>(k);
+    const int ldy = static_cast<int>((k + 15) / 16 * 16);  // row stride of rotation coefficients
+    const bool exact = complement == 2;
+
+    // Workspace: host inputs and outputs are staged in it; device ones are used in place.
+    const int gb = gram_blocks(std::max(nr, nc));
+    const int cb = std::max(cgs_blocks(nr), cgs_blocks(nc));
+    auto plan = [&](Carve& c, std::vector<double*>& p) {
+        p.clear();
+        p.push_back(device_ptrs ? nullptr : c.take<double>(nr * k));  // 0 U
+        p.push_back(device_ptrs ? nullptr : c.take<double>(nc * k));  // 1 V
+        p.push_back(device_ptrs ? nullptr : c.take<double>(nr * k));  // 2 G_U
+        p.push_back(device_ptrs ? nullptr : c.take<double>(nc * k));  // 3 G_V
+        p.push_back(device_ptrs ? nullptr : c.take<double>(nr * R));  // 4 X
+        p.push_back(device_ptrs ? nullptr : c.take<double>(nc * R));  // 5 Y
+        p.push_back(c.take<double>(static_cast<i64>(std::max(gb, cb)) * std::max<i64>(k * k, R)));  // 6 partials
+        p.push_back(c.take<double>(k * ldy));      // 7 J (rotation layout)
+        p.push_back(c.take<double>(k * ldy));      // 8 K
+        p.push_back(c.take<double>(2 * k * ldy + 2 * k));  // 9 coefficient staging: CX, CY, d, sig2
+        p.push_back(dcenter ? c.take<double>(nr + R + nc) : nullptr);  // 10 ones, column sums, centre gradient
+        if (exact) {
+            for (int side = 0; side < 2; ++side) {
+                const i64 L = side == 0 ? nr : nc, Lo = side == 0 ? nc : nr;
+                for (int v = 0; v < 6; ++v) p.push_back(c.take<double>(L * k));  // b x r p ap uc
+                p.push_back(c.take<double>(Lo * k));                            // tmp
+                p.push_back(c.take<double>(k * ldy));                           // proj
+                p.push_back(c.take<double>(6 * k));                             // rr rr_new pap alpha beta (+1)
+            }
+        }
     };
-    auto dU = up(U, nr * k), dV = up(V, nc * k), dGU = up(GU, nr * k), dGV = up(GV, nc * k);
+    std::vector<double*> wp;
+    Carve sizing;
+    plan(sizing, wp);
+    Carve carve;
+    carve.base = ctx->workspace(sizing.off);
+    plan(carve, wp);
+
+    const double *dU = U, *dV = V, *dGU = GU, *dGV = GV;
+    double *Xo = X, *Yo = Y, *dc = dcenter;
+    if (!device_ptrs) {
+        auto up = [&](double* dst, const double* src, i64 n) {
+            IRLBA_HIP_CHECK(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            return dst;
+        };
+        dU = up(wp[0], U, nr * k);
+        dV = up(wp[1], V, nc * k);
+        dGU = up(wp[2], GU, nr * k);
+        dGV = up(wp[3], GV, nc * k);
+        Xo = wp[4];
+        Yo = wp[5];
+        dc = dcenter ? wp[10] + nr + R : nullptr;
+    }
+    double* partials = wp[6];
+    double *dJ = wp[7], *dK = wp[8], *coef = wp[9];
     std::vector<double> hs(k), hw(k);
     if (device_ptrs) {
         IRLBA_HIP_CHECK(hipMemcpyAsync(hs.data(), sv, sizeof(double) * k, hipMemcpyDeviceToHost, s));
@@ -1192,103 +1263,132 @@ int run_svd_backward(const irlba_hip_matrix* m, i64 k, const double* U, const do
         std::copy(sv, sv + k, hs.begin());
         std::copy(w, w + k, hw.begin());
     }
-    // J = U'G_U, K = V'G_V (k x k, column i = basis' g_i).
-    auto dJ = dalloc<double>(k * k), dK = dalloc<double>(k * k);
-    for (i64 i = 0; i < k; ++i) {
-        bw.dots(dU.get(), nr, k, dGU.get() + i * nr, dJ.get() + i * k);
-        bw.dots(dV.get(), nc, k, dGV.get() + i * nc, dK.get() + i * k);
-    }
-    std::vector<double> J(k * k), K(k * k);
-    IRLBA_HIP_CHECK(hipMemcpyAsync(J.data(), dJ.get(), sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
-    IRLBA_HIP_CHECK(hipMemcpyAsync(K.data(), dK.get(), sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
+    // J = U'G_U, K = V'G_V, laid out as rotation coefficients (J(i, j) at i * ldy + j).
+    launch_gram(dU, nr, dGU, nr, ki, nr, partials, dJ, ldy, s);
+    launch_gram(dV, nc, dGV, nc, ki, nc, partials, dK, ldy, s);
+    std::vector<double> J(k * ldy), K(k * ldy);
+    IRLBA_HIP_CHECK(hipMemcpyAsync(J.data(), dJ, sizeof(double) * k * ldy, hipMemcpyDeviceToHost, s));
+    IRLBA_HIP_CHECK(hipMemcpyAsync(K.data(), dK, sizeof(double) * k * ldy, hipMemcpyDeviceToHost, s));
     IRLBA_HIP_CHECK(hipStreamSynchronize(s));
     for (i64 i = 0; i < k; ++i)
         require(hs[i] > 0 && (i == 0 || hs[i] <= hs[i - 1]), "singular values must be positive and non-increasing");
-    // M on the host.
-    HostMat M(k, k);
-    for (i64 j = 0; j < k; ++j)
-        for (i64 i = 0; i < k; ++i) {
+
+    // Host: CX = M (- J S^-1 in mode 1), CY = -K S^-1 (mode 1), d = 1 / s, sig2 = s^2.
+    std::vector<double> hc(static_cast<size_t>(2 * k * ldy + 2 * k), 0.0);
+    double *CX = hc.data(), *CY = hc.data() + k * ldy, *hd = CY + k * ldy, *hsig2 = hd + k;
+    for (i64 i = 0; i < k; ++i)
+        for (i64 j = 0; j < k; ++j) {
+            double mij = 0;
             if (i == j) {
-                M(i, i) = hw[i];
-                continue;
+                mij = hw[i];
+            } else {
+                const double den = hs[j] * hs[j] - hs[i] * hs[i];
+                const double F = den != 0 ? 1.0 / den : 0.0;  // repeated values: undefined gradient, term dropped
+                const double ju = J[i * ldy + j] - J[j * ldy + i], kv = K[i * ldy + j] - K[j * ldy + i];
+                mij = F * ju * hs[j] + hs[i] * F * kv;
             }
-            const double den = hs[j] * hs[j] - hs[i] * hs[i];
-            const double F = den != 0 ? 1.0 / den : 0.0;  // repeated values: the gradient is undefined; drop the term
-            const double ju = J[i + j * k] - J[j + i * k], kv = K[i + j * k] - K[j + i * k];
-            M(i, j) = F * ju * hs[j] + hs[i] * F * kv;
+            CX[i * ldy + j] = mij - (complement == 1 ? J[i * ldy + j] / hs[j] : 0.0);
+            CY[i * ldy + j] = complement == 1 ? -K[i * ldy + j] / hs[j] : 0.0;
         }
-    auto dX = dalloc<double>(nr * R), dY = dalloc<double>(nc * R);
-    // out (L x cols) = A[:, :Ym.nr] Ym on the matrix cores (the restart-rotation kernel).
-    auto rotate_into = [&](const double* A, i64 L, const HostMat& Ym, double* out) {
-        const i64 inner = Ym.nr, cols = Ym.nc;
-        const int ldy = static_cast<int>((cols + 15) / 16 * 16);
-        std::vector<double> yt(static_cast<size_t>(inner * ldy), 0.0);
-        for (i64 r = 0; r < inner; ++r)
-            for (i64 c = 0; c < cols; ++c) yt[r * ldy + c] = Ym(r, c);
-        auto dYt = dalloc<double>(inner * ldy);
-        IRLBA_HIP_CHECK(hipMemcpyAsync(dYt.get(), yt.data(), sizeof(double) * yt.size(), hipMemcpyHostToDevice, s));
-        launch_rotate(A, L, static_cast<int>(inner), dYt.get(), ldy, static_cast<int>(cols), out, L, L, s);
-        IRLBA_HIP_CHECK(hipStreamSynchronize(s));  // before the staging buffers go out of scope
-    };
-    // X[:, :k] = U M, X[:, k:] = U; Y[:, :k] = V.
-    rotate_into(dU.get(), nr, M, dX.get());
-    IRLBA_HIP_CHECK(hipMemcpyAsync(dX.get() + nr * k, dU.get(), sizeof(double) * nr * k, hipMemcpyDeviceToDevice, s));
-    IRLBA_HIP_CHECK(hipMemcpyAsync(dY.get(), dV.get(), sizeof(double) * nc * k, hipMemcpyDeviceToDevice, s));
-    double* Zc = dX.get();            // accumulate Z' into X[:, :k]
-    double* Yc = dY.get() + nc * k;   // Y'
-    IRLBA_HIP_CHECK(hipMemsetAsync(Yc, 0, sizeof(double) * nc * k, s));
-    int cg_total = 0;
-    if (complement >= 1) {
-        // P g_i = g_i - U J_i and Q h_i = h_i - V K_i.
-        HostMat Jm(k, k), Km(k, k);
-        std::copy(J.begin(), J.end(), Jm.a.begin());
-        std::copy(K.begin(), K.end(), Km.a.begin());
-        auto PG = dalloc<double>(nr * k), QH = dalloc<double>(nc * k);
-        rotate_into(dU.get(), nr, Jm, PG.get());
-        launch_axpby(PG.get(), dGU.get(), 1.0, PG.get(), -1.0, nr * k, s);
-        rotate_into(dV.get(), nc, Km, QH.get());
-        launch_axpby(QH.get(), dGV.get(), 1.0, QH.get(), -1.0, nc * k, s);
-        if (complement == 1) {
-            for (i64 i = 0; i < k; ++i) {
-                launch_axpby(Zc + i * nr, Zc + i * nr, 1.0, PG.get() + i * nr, 1.0 / hs[i], nr, s);
-                launch_axpby(Yc + i * nc, QH.get() + i * nc, 1.0 / hs[i], nullptr, 0.0, nc, s);
-            }
-        } else {
-            auto z = dalloc<double>(nr), t = dalloc<double>(nc), a = dalloc<double>(std::max(nr, nc));
-            for (i64 i = 0; i < k; ++i) {
-                const double sig = hs[i], sig2 = sig * sig;
-                cg_total += bw.cg(true, sig2, dU.get(), PG.get() + i * nr, z.get(), max_cg, cg_tol);
-                cg_total += bw.cg(false, sig2, dV.get(), QH.get() + i * nc, t.get(), max_cg, cg_tol);
-                // Z'_i = s z + A t ; Y'_i = A' z + s t
-                apply_handle(*m, false, t.get(), a.get(), nullptr, nullptr);
-                launch_axpby(Zc + i * nr, Zc + i * nr, 1.0, z.get(), sig, nr, s);
-                launch_axpby(Zc + i * nr, Zc + i * nr, 1.0, a.get(), 1.0, nr, s);
-                apply_handle(*m, true, z.get(), a.get(), nullptr, nullptr);
-                launch_axpby(Yc + i * nc, a.get(), 1.0, t.get(), sig, nc, s);
-            }
-        }
+    for (i64 i = 0; i < k; ++i) {
+        hd[i] = 1.0 / hs[i];
+        hsig2[i] = hs[i] * hs[i];
     }
-    // Outputs.
-    const auto out_kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-    IRLBA_HIP_CHECK(hipMemcpyAsync(X, dX.get(), sizeof(double) * nr * R, out_kind, s));
-    IRLBA_HIP_CHECK(hipMemcpyAsync(Y, dY.get(), sizeof(double) * nc * R, out_kind, s));
+    IRLBA_HIP_CHECK(hipMemcpyAsync(coef, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice, s));
+    const double *dCX = coef, *dCY = coef + k * ldy, *dd = dCY + k * ldy, *dsig2 = dd + k;
+
+    // X = [U | U CX (+ G_U S^-1)], Y = [Y' | V].
+    double *X0 = Xo, *X1 = Xo + nr * k, *Y0 = Yo, *Y1 = Yo + nc * k;
+    IRLBA_HIP_CHECK(hipMemcpyAsync(X0, dU, sizeof(double) * nr * k, hipMemcpyDeviceToDevice, s));
+    IRLBA_HIP_CHECK(hipMemcpyAsync(Y1, dV, sizeof(double) * nc * k, hipMemcpyDeviceToDevice, s));
+    launch_rotate(dU, nr, ki, dCX, ldy, ki, X1, nr, nr, s);
+    int iterations = 0;
+    if (complement == 0) {
+        launch_fill(Y0, 0.0, nc * k, s);
+    } else if (complement == 1) {
+        launch_cols_axpby(X1, nr, nullptr, 1.0, dGU, nr, dd, 1.0, ki, nr, s);
+        launch_rotate(dV, nc, ki, dCY, ldy, ki, Y0, nc, nc, s);
+        launch_cols_axpby(Y0, nc, nullptr, 1.0, dGV, nc, dd, 1.0, ki, nc, s);
+    } else {
+        LockstepCg cg{m, s, ki, ldy};
+        CgSet set[2];
+        size_t at = 11;
+        for (int side = 0; side < 2; ++side) {
+            CgSet& c = set[side];
+            c.left = side == 0;
+            c.L = c.left ? nr : nc;
+            c.Lo = c.left ? nc : nr;
+            c.basis = c.left ? dU : dV;
+            c.b = wp[at++];
+            c.x = wp[at++];
+            c.r = wp[at++];
+            c.p = wp[at++];
+            c.ap = wp[at++];
+            c.uc = wp[at++];
+            c.tmp = wp[at++];
+            c.proj = wp[at++];
+            double* sc = wp[at++];
+            c.rr = sc;
+            c.rr_new = sc + k;
+            c.pap = sc + 2 * k;
+            c.alpha = sc + 3 * k;
+            c.beta = sc + 4 * k;
+            c.sig2 = const_cast<double*>(dsig2);
+            c.partials = partials;
+            // b = P g = g - basis (basis' g); basis' g is J (left) or K (right).
+            launch_rotate(c.basis, c.L, ki, c.left ? dJ : dK, ldy, ki, c.b, c.L, c.L, s);
+            launch_cols_axpby(c.b, c.L, nullptr, -1.0, c.left ? dGU : dGV, c.L, nullptr, 1.0, ki, c.L, s);
+            cg.start(c);
+        }
+        // Convergence: |r_i| <= tol |b_i| for every system of both sides.
+        std::vector<double> bb(2 * k), rr(2 * k);
+        double* bnorm = wp[at - 1] + 5 * k;  // spare slot of the right set's scalars (k entries)
+        launch_coldots(set[0].b, nr, set[0].b, nr, ki, nr, partials, set[0].beta, s);
+        launch_coldots(set[1].b, nc, set[1].b, nc, ki, nc, partials, bnorm, s);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(bb.data(), set[0].beta, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        IRLBA_HIP_CHECK(hipMemcpyAsync(bb.data() + k, bnorm, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        auto converged = [&]() {
+            IRLBA_HIP_CHECK(hipMemcpyAsync(rr.data(), set[0].rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+            IRLBA_HIP_CHECK(hipMemcpyAsync(rr.data() + k, set[1].rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+            IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+            for (i64 i = 0; i < 2 * k; ++i)
+                if (!(rr[i] <= cg_tol * cg_tol * bb[i])) return false;
+            return true;
+        };
+        while (iterations < max_cg && !converged()) {
+            cg.step(set[0]);
+            cg.step(set[1]);
+            ++iterations;
+        }
+        cg.finish(set[0]);
+        cg.finish(set[1]);
+        // Z' = Z S + A T into X1;  Y' = A' Z + T S into Y0.  The right set's
+        // scratch (rows x k) receives A T.
+        for (i64 i = 0; i < k; ++i) {
+            apply_handle(*m, false, set[1].x + i * nc, set[1].tmp + i * nr, nullptr, nullptr);
+            apply_handle(*m, true, set[0].x + i * nr, Y0 + i * nc, nullptr, nullptr);
+        }
+        launch_cols_axpby(X1, nr, nullptr, 1.0, set[1].tmp, nr, nullptr, 1.0, ki, nr, s);
+        launch_cg_coefs(dsig2, nullptr, set[0].alpha, ki, 2, s);  // alpha = sqrt(sig2) = s
+        launch_cols_axpby(X1, nr, nullptr, 1.0, set[0].x, nr, set[0].alpha, 1.0, ki, nr, s);
+        launch_cols_axpby(Y0, nc, nullptr, 1.0, set[1].x, nc, set[0].alpha, 1.0, ki, nc, s);
+    }
     if (dcenter) {
         // d/dc of A - 1 c' is -(X Y')' 1 = -Y (X' 1).
-        const std::vector<double> ho(static_cast<size_t>(nr), 1.0);
-        auto ones = dalloc<double>(nr), xs = dalloc<double>(R), dc = dalloc<double>(nc);
-        IRLBA_HIP_CHECK(hipMemcpyAsync(ones.get(), ho.data(), sizeof(double) * nr, hipMemcpyHostToDevice, s));
-        bw.dots(dX.get(), nr, R, ones.get(), xs.get());
-        std::vector<double> hx(R);
-        IRLBA_HIP_CHECK(hipMemcpyAsync(hx.data(), xs.get(), sizeof(double) * R, hipMemcpyDeviceToHost, s));
-        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
-        HostMat col(R, 1);
-        for (i64 r = 0; r < R; ++r) col(r, 0) = -hx[r];
-        rotate_into(dY.get(), nc, col, dc.get());
-        IRLBA_HIP_CHECK(hipMemcpyAsync(dcenter, dc.get(), sizeof(double) * nc, out_kind, s));
-        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+        double *ones = wp[10], *xs = wp[10] + nr;
+        launch_fill(ones, 1.0, nr, s);
+        launch_cgs_dots(Xo, nr, static_cast<int>(R), ones, nr, partials, s);
+        launch_cgs_reduce(partials, nr, static_cast<int>(R), xs, s);
+        launch_fill(dc, 0.0, nc, s);
+        launch_cgs_update(Yo, nc, static_cast<int>(R), xs, dc, nc, partials, s);
+    }
+    if (!device_ptrs) {
+        IRLBA_HIP_CHECK(hipMemcpyAsync(X, Xo, sizeof(double) * nr * R, hipMemcpyDeviceToHost, s));
+        IRLBA_HIP_CHECK(hipMemcpyAsync(Y, Yo, sizeof(double) * nc * R, hipMemcpyDeviceToHost, s));
+        if (dcenter) IRLBA_HIP_CHECK(hipMemcpyAsync(dcenter, dc, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
     }
     IRLBA_HIP_CHECK(hipStreamSynchronize(s));
-    return cg_total;
+    return iterations;
 }
 
 }  // namespace
@@ -1799,44 +1899,77 @@ int irlba_hip_svd_backward(const irlba_hip_matrix* m, int64_t k, const double* U
 
 int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_t* indices,
                     const int64_t* pointers, int column_major, int64_t rank, const double* X, const double* Y,
-                    double* out, int device_pointers) {
+                    void* out, int out_type, int device_pointers) {
     return guarded([&] {
         require(ctx && pointers && X && Y, "null argument");
         require(nrow >= 0 && ncol >= 0, "negative dimension");
         require(rank >= 1 && rank <= 128, "rank must lie in [1, 128]");
+        require(out_type == IRLBA_HIP_F64 || out_type == IRLBA_HIP_F32, "out_type must be F64 or F32");
         DeviceGuard g(ctx->device);
         hipStream_t s = ctx->stream;
+        const bool f32 = out_type == IRLBA_HIP_F32;
+        const size_t ob = f32 ? 4 : 8;
         const i64 n_lines = column_major ? ncol : nrow, n_other = column_major ? nrow : ncol;
         i64 nnz = 0;
-        IRLBA_HIP_CHECK(hipMemcpy(&nnz, pointers + n_lines, sizeof(i64),
-                                  device_pointers ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+        IRLBA_HIP_CHECK(hipMemcpyAsync(&nnz, pointers + n_lines, sizeof(i64),
+                                       device_pointers ? hipMemcpyDeviceToHost : hipMemcpyHostToHost, s));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
         require(nnz >= 0, "pointers are not a valid compressed layout");
         require(nnz == 0 || (indices && out), "null argument");
-        const auto in_kind = device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-        auto dptr = dalloc<i64>(n_lines + 1);
-        auto didx = dalloc<std::int32_t>(nnz);
-        auto dX = dalloc<double>(nrow * rank), dY = dalloc<double>(ncol * rank);
-        auto drm = dalloc<double>(n_other * rank), dout = dalloc<double>(nnz);
-        IRLBA_HIP_CHECK(hipMemcpyAsync(dptr.get(), pointers, sizeof(i64) * (n_lines + 1), in_kind, s));
-        if (nnz) IRLBA_HIP_CHECK(hipMemcpyAsync(didx.get(), indices, sizeof(std::int32_t) * nnz, in_kind, s));
-        IRLBA_HIP_CHECK(hipMemcpyAsync(dX.get(), X, sizeof(double) * nrow * rank, in_kind, s));
-        IRLBA_HIP_CHECK(hipMemcpyAsync(dY.get(), Y, sizeof(double) * ncol * rank, in_kind, s));
+        const int Rp = sddmm_padded_rank(static_cast<int>(rank), f32);
+        // Device arrays are used in place; host ones are staged in the workspace.
+        auto plan = [&](Carve& c, std::vector<void*>& p) {
+            p.clear();
+            p.push_back(c.take<int>(1));                                           // 0 flag
+            p.push_back(c.take<char>(static_cast<i64>(ob) * Rp * (nrow + ncol)));  // 1 packed factors
+            p.push_back(device_pointers ? nullptr : c.take<i64>(n_lines + 1));     // 2 pointers
+            p.push_back(device_pointers ? nullptr : c.take<std::int32_t>(nnz));    // 3 indices
+            p.push_back(device_pointers ? nullptr : c.take<double>(nrow * rank));  // 4 X
+            p.push_back(device_pointers ? nullptr : c.take<double>(ncol * rank));  // 5 Y
+            p.push_back(device_pointers ? nullptr : c.take<char>(static_cast<i64>(ob) * nnz));  // 6 out
+        };
+        std::vector<void*> wp;
+        Carve sizing;
+        plan(sizing, wp);
+        Carve carve;
+        carve.base = ctx->workspace(sizing.off);
+        plan(carve, wp);
+        const i64* dptr = pointers;
+        const std::int32_t* didx = indices;
+        const double *dX = X, *dY = Y;
+        void* dout = out;
+        if (!device_pointers) {
+            IRLBA_HIP_CHECK(hipMemcpyAsync(wp[2], pointers, sizeof(i64) * (n_lines + 1), hipMemcpyHostToDevice, s));
+            if (nnz)
+                IRLBA_HIP_CHECK(
+                    hipMemcpyAsync(wp[3], indices, sizeof(std::int32_t) * nnz, hipMemcpyHostToDevice, s));
+            IRLBA_HIP_CHECK(hipMemcpyAsync(wp[4], X, sizeof(double) * nrow * rank, hipMemcpyHostToDevice, s));
+            IRLBA_HIP_CHECK(hipMemcpyAsync(wp[5], Y, sizeof(double) * ncol * rank, hipMemcpyHostToDevice, s));
+            dptr = static_cast<const i64*>(wp[2]);
+            didx = static_cast<const std::int32_t*>(wp[3]);
+            dX = static_cast<const double*>(wp[4]);
+            dY = static_cast<const double*>(wp[5]);
+            dout = wp[6];
+        }
         // The same checks as matrix creation: monotone pointers, indices in range.
-        auto flag = dalloc<int>(1);
-        IRLBA_HIP_CHECK(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
-        launch_validate_compressed(didx.get(), dptr.get(), n_lines, n_other, nnz, flag.get(), s);
+        int* flag = static_cast<int*>(wp[0]);
+        IRLBA_HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+        launch_validate_compressed(didx, dptr, n_lines, n_other, nnz, flag, s);
         int hflag = 0;
-        IRLBA_HIP_CHECK(hipMemcpyAsync(&hflag, flag.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        IRLBA_HIP_CHECK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
         IRLBA_HIP_CHECK(hipStreamSynchronize(s));
         require(!(hflag & 1), "pointers are not a valid compressed layout");
         require(!(hflag & 2), "indices out of range");
-        const double* line_f = column_major ? dY.get() : dX.get();
-        const double* entry_f = column_major ? dX.get() : dY.get();
-        launch_to_row_major(entry_f, n_other, n_other, static_cast<int>(rank), drm.get(), s);
-        launch_sddmm(didx.get(), dptr.get(), n_lines, line_f, drm.get(), static_cast<int>(rank), dout.get(), s);
-        if (nnz)
-            IRLBA_HIP_CHECK(hipMemcpyAsync(out, dout.get(), sizeof(double) * nnz,
-                                           device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+        // Both factors packed row-major (stride Rp) in the output precision.
+        char* packed = static_cast<char*>(wp[1]);
+        char* pX = packed;
+        char* pY = packed + ob * Rp * nrow;
+        launch_pack_rows(dX, nrow, nrow, static_cast<int>(rank), Rp, f32, pX, s);
+        launch_pack_rows(dY, ncol, ncol, static_cast<int>(rank), Rp, f32, pY, s);
+        launch_sddmm(didx, dptr, n_lines, column_major ? pY : pX, column_major ? pX : pY, static_cast<int>(rank), f32,
+                     dout, s);
+        if (!device_pointers && nnz)
+            IRLBA_HIP_CHECK(hipMemcpyAsync(out, dout, ob * nnz, hipMemcpyDeviceToHost, s));
         IRLBA_HIP_CHECK(hipStreamSynchronize(s));
     });
 }

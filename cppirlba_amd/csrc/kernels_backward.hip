@@ -3,73 +3,291 @@
 // The adjoint of L = sum_i w_i s_i + <G_U, U> + <G_V, V> with respect to the
 // matrix is low rank, dL/dA = X Y' (X rows x R, Y cols x R, R = 2k).  For a
 // sparse matrix only the stored entries need it: out[e] = X[row_e, :] . Y[col_e, :]
-// (an SDDMM).  One wavefront per line of the compressed pattern: the line's
-// factor row is staged once in LDS, each lane takes entries of the line and
-// reads the matching row of the other factor, stored row-major so that it is
-// one contiguous R*8-byte run.
+// (an SDDMM).  Both factors are first packed row-major (row stride Rp, zero
+// padded, in the precision of the output), so that one stored entry needs one
+// contiguous Rp-element run of the other factor: LPE lanes of a wavefront
+// take one entry, each loading one 16-byte piece of the run, and reduce with
+// shuffles.  A wavefront walks one line (column of a CSC) with the line's own
+// factor row held in registers.
+//
+// The small dense products of the backward (U'G, the centre sums) are
+// tall-skinny Gram products A'B with both operands L x k, k <= 64: blocks over
+// row ranges stage 32-row tiles of both in LDS and write k x k partials that a
+// second kernel sums in a fixed order.
 #include "common.hpp"
 
 namespace irlba_hip {
 
-constexpr int kSddmmMaxR = 128;
-
-// X_rm[row * R + r] = X[row + r * ld]
-__global__ void k_to_row_major(const double* __restrict__ X, i64 rows, i64 ld, int R, double* __restrict__ out) {
-    const i64 n = rows * R;
+// ------------------------------------------------------------------ packing
+// out[row * Rp + r] = X[row + r * ld] for r < R, 0 for R <= r < Rp.
+template <typename FT>
+__global__ void k_pack_rows(const double* __restrict__ X, i64 rows, i64 ld, int R, int Rp, FT* __restrict__ out) {
+    const i64 n = rows * Rp;
     for (i64 e = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
          e += static_cast<i64>(gridDim.x) * blockDim.x) {
-        const i64 row = e / R;
-        const int r = static_cast<int>(e - row * R);
-        out[e] = X[row + static_cast<i64>(r) * ld];
+        const i64 row = e / Rp;
+        const int r = static_cast<int>(e - row * Rp);
+        out[e] = r < R ? static_cast<FT>(X[row + static_cast<i64>(r) * ld]) : FT(0);
     }
 }
 
-void launch_to_row_major(const double* X, i64 rows, i64 ld, int R, double* out, hipStream_t s) {
-    if (rows <= 0 || R <= 0) return;
-    const i64 grid = std::max<i64>(1, std::min<i64>((rows * R + 255) / 256, 16384));
-    hipLaunchKernelGGL(k_to_row_major, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, X, rows, ld, R, out);
+void launch_pack_rows(const double* X, i64 rows, i64 ld, int R, int Rp, bool f32, void* out, hipStream_t s) {
+    if (rows <= 0 || Rp <= 0) return;
+    const i64 grid = std::max<i64>(1, std::min<i64>((rows * Rp + 255) / 256, 16384));
+    if (f32)
+        hipLaunchKernelGGL(k_pack_rows<float>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, X, rows, ld, R, Rp,
+                           static_cast<float*>(out));
+    else
+        hipLaunchKernelGGL(k_pack_rows<double>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, X, rows, ld, R, Rp,
+                           static_cast<double*>(out));
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
-// Lines are the primary dimension of the pattern (columns of a CSC); `line_f`
-// is the factor indexed by the line (column-major, ld = n_lines), `entry_f`
-// the factor indexed by the stored index, row-major (R per row).
+// ------------------------------------------------------------------ SDDMM
+template <typename FT>
+struct Piece;  // one 16-byte load
+template <>
+struct Piece<float> {
+    typedef f32x4 type;
+    static constexpr int n = 4;
+};
+template <>
+struct Piece<double> {
+    typedef f64x2 type;
+    static constexpr int n = 2;
+};
+
+template <typename FT, int LPE>
 __global__ __launch_bounds__(256) void k_sddmm(const std::int32_t* __restrict__ idx, const i64* __restrict__ ptr,
-                                               i64 n_lines, const double* __restrict__ line_f,
-                                               const double* __restrict__ entry_f_rm, int R,
-                                               double* __restrict__ out) {
-    __shared__ double yl[4][kSddmmMaxR];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const i64 wave = static_cast<i64>(blockIdx.x) * 4 + w;
-    const i64 nwaves = static_cast<i64>(gridDim.x) * 4;
+                                               i64 n_lines, const FT* __restrict__ line_rm,
+                                               const FT* __restrict__ entry_rm, int Rp, FT* __restrict__ out) {
+    typedef typename Piece<FT>::type PT;
+    constexpr int VN = Piece<FT>::n;
+    constexpr int G = kWave / LPE;  // entries in flight per wavefront
+    const int lane = threadIdx.x & 63, q = lane % LPE, g = lane / LPE;
+    const i64 wave = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const i64 nwaves = (static_cast<i64>(gridDim.x) * blockDim.x) >> 6;
     for (i64 c = wave; c < n_lines; c += nwaves) {
-        for (int r = lane; r < R; r += 64) yl[w][r] = line_f[c + static_cast<i64>(r) * n_lines];
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the staged row is visible to the wave
+        const PT yl = *reinterpret_cast<const PT*>(line_rm + c * Rp + q * VN);
         const i64 e0 = ptr[c], e1 = ptr[c + 1];
-        for (i64 e = e0 + lane; e < e1; e += 64) {
-            const double* xr = entry_f_rm + static_cast<i64>(idx[e]) * R;
-            double a0 = 0, a1 = 0;
-            int r = 0;
-            for (; r + 1 < R; r += 2) {
-                a0 += xr[r] * yl[w][r];
-                a1 += xr[r + 1] * yl[w][r + 1];
+        // Every lane runs the same trip count so the group shuffles stay converged.
+        for (i64 base = e0; base < e1; base += G) {
+            const i64 e = base + g;
+            double acc = 0;
+            if (e < e1) {
+                const i64 row = idx[e];
+                // A normal (cached) load: each factor row is shared by every entry of its row.
+                const PT x = *reinterpret_cast<const PT*>(entry_rm + row * Rp + q * VN);
+#pragma unroll
+                for (int v = 0; v < VN; ++v) acc += static_cast<double>(x[v]) * static_cast<double>(yl[v]);
             }
-            if (r < R) a0 += xr[r] * yl[w][r];
-            out[e] = a0 + a1;
+            acc = group_sum<LPE>(acc);
+            if (q == 0 && e < e1) out[e] = static_cast<FT>(acc);
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_s_waitcnt(0xc07f);
     }
 }
 
-void launch_sddmm(const std::int32_t* idx, const i64* ptr, i64 n_lines, const double* line_f,
-                  const double* entry_f_rm, int R, double* out, hipStream_t s) {
+template <typename FT, int LPE>
+static void sddmm_launch(const std::int32_t* idx, const i64* ptr, i64 n_lines, const void* line_rm,
+                         const void* entry_rm, int Rp, void* out, hipStream_t s) {
+    const i64 grid = std::max<i64>(1, std::min<i64>((n_lines + 3) / 4, 1 << 20));
+    hipLaunchKernelGGL((k_sddmm<FT, LPE>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, idx, ptr, n_lines,
+                       static_cast<const FT*>(line_rm), static_cast<const FT*>(entry_rm), Rp, static_cast<FT*>(out));
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+int sddmm_padded_rank(int R, bool f32, int* lpe_out) {
+    const int vn = f32 ? 4 : 2;
+    int lpe = 1;
+    while (lpe * vn < R) lpe *= 2;
+    if (lpe > 64) throw std::runtime_error("SDDMM rank too large");
+    if (lpe_out) *lpe_out = lpe;
+    return lpe * vn;
+}
+
+void launch_sddmm(const std::int32_t* idx, const i64* ptr, i64 n_lines, const void* line_rm, const void* entry_rm,
+                  int R, bool f32, void* out, hipStream_t s) {
     if (n_lines <= 0) return;
-    if (R > kSddmmMaxR) throw std::runtime_error("SDDMM rank above 128");
-    const i64 grid = std::max<i64>(1, std::min<i64>((n_lines + 3) / 4, 65536));
-    hipLaunchKernelGGL(k_sddmm, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, idx, ptr, n_lines, line_f,
-                       entry_f_rm, R, out);
+    int lpe = 1;
+    const int Rp = sddmm_padded_rank(R, f32, &lpe);
+    if (f32) {
+        switch (lpe) {
+            case 1: sddmm_launch<float, 1>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+            case 2: sddmm_launch<float, 2>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+            case 4: sddmm_launch<float, 4>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+            case 8: sddmm_launch<float, 8>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+            case 16: sddmm_launch<float, 16>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+            case 32: sddmm_launch<float, 32>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+            default: sddmm_launch<float, 64>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+        }
+    } else {
+        switch (lpe) {
+            case 1: sddmm_launch<double, 1>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+            case 2: sddmm_launch<double, 2>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+            case 4: sddmm_launch<double, 4>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+            case 8: sddmm_launch<double, 8>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+            case 16: sddmm_launch<double, 16>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+            case 32: sddmm_launch<double, 32>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+            default: sddmm_launch<double, 64>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Gram products A'B
+constexpr int kGramRows = 32;
+constexpr int kGramMaxK = 64;
+constexpr int kGramPad = kGramRows + 1;  // LDS row stride (breaks bank aliasing)
+
+int gram_blocks(i64 L) { return static_cast<int>(std::max<i64>(1, std::min<i64>(256, (L + 255) / 256))); }
+
+__global__ __launch_bounds__(256) void k_gram(const double* __restrict__ A, i64 lda, const double* __restrict__ B,
+                                              i64 ldb, int k, i64 L, i64 chunk, double* __restrict__ partials) {
+    __shared__ double As[kGramMaxK * kGramPad];
+    __shared__ double Bs[kGramMaxK * kGramPad];
+    const int t = threadIdx.x;
+    const int kk = k * k;
+    double acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0;
+    const i64 r0 = static_cast<i64>(blockIdx.x) * chunk, r1 = std::min<i64>(L, r0 + chunk);
+    for (i64 base = r0; base < r1; base += kGramRows) {
+        const int n = static_cast<int>(std::min<i64>(kGramRows, r1 - base));
+        // Columns are contiguous in HBM: consecutive threads take consecutive rows.
+        for (int e = t; e < kGramRows * k; e += 256) {
+            const int r = e % kGramRows, c = e / kGramRows;
+            As[c * kGramPad + r] = r < n ? A[base + r + static_cast<i64>(c) * lda] : 0.0;
+            Bs[c * kGramPad + r] = r < n ? B[base + r + static_cast<i64>(c) * ldb] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int p = t + 256 * j;
+            if (p < kk) {
+                const int i = p / k, jj = p - i * k;
+                const double* a = As + i * kGramPad;
+                const double* b = Bs + jj * kGramPad;
+                double s = 0;
+                for (int r = 0; r < n; ++r) s += a[r] * b[r];
+                acc[j] += s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int p = t + 256 * j;
+        if (p < kk) partials[static_cast<i64>(blockIdx.x) * kk + p] = acc[j];
+    }
+}
+
+// out[i * ld_out + j] = sum over blocks of partials[b][i * k + j] = (A'B)(i, j):
+// row-major, the layout of the rotation coefficients.
+__global__ void k_gram_reduce(const double* __restrict__ partials, int nb, int k, double* __restrict__ out,
+                              int ld_out) {
+    const int kk = k * k;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < kk; p += gridDim.x * blockDim.x) {
+        double s = 0;
+        for (int b = 0; b < nb; ++b) s += partials[static_cast<i64>(b) * kk + p];
+        const int i = p / k, j = p - i * k;
+        out[static_cast<i64>(i) * ld_out + j] = s;
+    }
+}
+
+void launch_gram(const double* A, i64 lda, const double* B, i64 ldb, int k, i64 L, double* partials, double* out,
+                 int ld_out, hipStream_t s) {
+    if (k <= 0) return;
+    if (k > kGramMaxK) throw std::runtime_error("Gram product wider than 64 columns");
+    const int nb = gram_blocks(L);
+    const i64 chunk = (L + nb - 1) / nb;
+    hipLaunchKernelGGL(k_gram, dim3(nb), dim3(256), 0, s, A, lda, B, ldb, k, L, chunk, partials);
+    IRLBA_HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gram_reduce, dim3((k * k + 255) / 256), dim3(256), 0, s, partials, nb, k, out, ld_out);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ column-batched vector kernels
+// out[i] = A[:, i] . B[:, i] for i < k: blocks (b, i) over row chunks, then a
+// fixed-order sum of the gram_blocks(L) partials of each column.
+__global__ __launch_bounds__(256) void k_coldots(const double* __restrict__ A, i64 lda, const double* __restrict__ B,
+                                                 i64 ldb, i64 L, i64 chunk, double* __restrict__ partials) {
+    __shared__ double red[8];
+    const int i = blockIdx.y;
+    const i64 r0 = static_cast<i64>(blockIdx.x) * chunk, r1 = std::min<i64>(L, r0 + chunk);
+    const double* a = A + static_cast<i64>(i) * lda;
+    const double* b = B + static_cast<i64>(i) * ldb;
+    double s = 0, unused = 0;
+    for (i64 r = r0 + threadIdx.x; r < r1; r += blockDim.x) s += a[r] * b[r];
+    block_sum2(s, unused, red);
+    if (threadIdx.x == 0) partials[static_cast<i64>(i) * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void k_coldots_reduce(const double* __restrict__ partials, int nb, int k, double* __restrict__ out) {
+    __shared__ double red[8];
+    const int i = blockIdx.x;
+    double s = 0, unused = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partials[static_cast<i64>(i) * nb + b];
+    block_sum2(s, unused, red);
+    if (threadIdx.x == 0) out[i] = s;
+}
+
+void launch_coldots(const double* A, i64 lda, const double* B, i64 ldb, int k, i64 L, double* partials, double* out,
+                    hipStream_t s) {
+    if (k <= 0) return;
+    const int nb = gram_blocks(L);
+    const i64 chunk = (L + nb - 1) / nb;
+    hipLaunchKernelGGL(k_coldots, dim3(nb, k), dim3(256), 0, s, A, lda, B, ldb, L, chunk, partials);
+    IRLBA_HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_coldots_reduce, dim3(k), dim3(256), 0, s, partials, nb, k, out);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+// out[:, i] = out[:, i] * (a_scale * a_coef[i]) + x[:, i] * (b_scale * b_coef[i])
+// for i < k (a null coefficient array reads as ones).  With a_coef null and
+// a_scale 0 the old contents of out are not read (out may be uninitialised).
+__global__ void k_cols_axpby(double* __restrict__ out, i64 ldo, const double* __restrict__ a_coef, double a_scale,
+                             const double* __restrict__ x, i64 ldx, const double* __restrict__ b_coef,
+                             double b_scale, int k, i64 L) {
+    const i64 n = L * k;
+    const bool overwrite = a_coef == nullptr && a_scale == 0.0;
+    for (i64 e = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<i64>(gridDim.x) * blockDim.x) {
+        const int i = static_cast<int>(e / L);
+        const i64 l = e - static_cast<i64>(i) * L;
+        double& o = out[l + static_cast<i64>(i) * ldo];
+        const double bt = x[l + static_cast<i64>(i) * ldx] * (b_coef ? b_coef[i] * b_scale : b_scale);
+        o = overwrite ? bt : o * (a_coef ? a_coef[i] * a_scale : a_scale) + bt;
+    }
+}
+
+void launch_cols_axpby(double* out, i64 ldo, const double* a_coef, double a_scale, const double* x, i64 ldx,
+                       const double* b_coef, double b_scale, int k, i64 L, hipStream_t s) {
+    if (L <= 0 || k <= 0) return;
+    const i64 grid = std::max<i64>(1, std::min<i64>((L * k + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_cols_axpby, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, out, ldo, a_coef, a_scale, x,
+                       ldx, b_coef, b_scale, k, L);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+// The scalar recurrences of the lock-step CG, one thread per system:
+//   mode 0: out = a / b (0 unless b > 0)          (alpha = rr / p'Ap)
+//   mode 1: out = a / b (0 unless b > 0); b = a    (beta = rr_new / rr; rr = rr_new)
+//   mode 2: out = sqrt(a);  mode 3: out = 1 / a.
+__global__ void k_cg_coefs(const double* a, double* b, double* out, int k, int mode) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    const double av = a[i];
+    if (mode <= 1) {
+        const double bv = b[i];
+        out[i] = bv > 0 ? av / bv : 0.0;
+        if (mode == 1) b[i] = av;
+    } else {
+        out[i] = mode == 2 ? sqrt(av) : 1.0 / av;
+    }
+}
+
+void launch_cg_coefs(const double* a, double* b, double* out, int k, int mode, hipStream_t s) {
+    if (k <= 0) return;
+    hipLaunchKernelGGL(k_cg_coefs, dim3((k + 63) / 64), dim3(64), 0, s, a, b, out, k, mode);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
@@ -86,6 +304,20 @@ void launch_axpby(double* out, const double* a, double alpha, const double* b, d
     if (n <= 0) return;
     const i64 grid = std::max<i64>(1, std::min<i64>((n + 255) / 256, 4096));
     hipLaunchKernelGGL(k_axpby, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, out, a, alpha, b, beta, n);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+// out[i] = v for i < n.
+__global__ void k_fill(double* __restrict__ out, double v, i64 n) {
+    for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * blockDim.x)
+        out[i] = v;
+}
+
+void launch_fill(double* out, double v, i64 n, hipStream_t s) {
+    if (n <= 0) return;
+    const i64 grid = std::max<i64>(1, std::min<i64>((n + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_fill, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, out, v, n);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 

@@ -194,12 +194,17 @@ public:
             auto h = ctx->saved_data["handles"].toCustomClass<Handles>();
             const int dev = static_cast<int>(ctx->saved_data["dev"].toInt());
             at::Tensor i = ctx->saved_data["indices"].toTensor(), pp = ctx->saved_data["pointers"].toTensor();
-            dvalues = at::empty({i.numel()}, f64_on(dev));
+            // The gradient comes out in the values' own type (FP32 values: FP32
+            // factors with FP64 dot products).
+            const auto vt = static_cast<at::ScalarType>(ctx->saved_data["values_dtype"].toInt());
+            const bool f32 = vt == at::kFloat;
+            dvalues = at::empty({i.numel()}, f64_on(dev).dtype(f32 ? at::kFloat : at::kDouble));
             check(irlba_hip_sddmm(h->ctx, ctx->saved_data["nrow"].toInt(), ctx->saved_data["ncol"].toInt(),
                                   i.data_ptr<int32_t>(), pp.data_ptr<int64_t>(),
                                   ctx->saved_data["column_major"].toBool(), f.X.size(0), f.X.data_ptr<double>(),
-                                  f.Y.data_ptr<double>(), dvalues.data_ptr<double>(), dev >= 0));
-            dvalues = dvalues.to(static_cast<at::ScalarType>(ctx->saved_data["values_dtype"].toInt()));
+                                  f.Y.data_ptr<double>(), dvalues.data_ptr(), f32 ? IRLBA_HIP_F32 : IRLBA_HIP_F64,
+                                  dev >= 0));
+            if (dvalues.scalar_type() != vt) dvalues = dvalues.to(vt);
         }
         return {dvalues, at::Tensor(), at::Tensor(), at::Tensor(), at::Tensor(), at::Tensor(), f.dcenter,
                 at::Tensor(), at::Tensor()};

@@ -178,16 +178,18 @@ int irlba_hip_generate_csc(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, d
  * reference computes the forward only).  Given the k triplets of m (U rows x k,
  * D k, V cols x k, e.g. from irlba_hip_compute) and the gradients of a loss
  * with respect to them (grad_D k, grad_U rows x k, grad_V cols x k; NULL =
- * zero), writes dL/dA = X Y' in factored form: X rows x 2k, Y cols x 2k.
+ * zero), writes dL/dA = X Y' in factored form: X rows x 2k, Y cols x 2k,
+ * X = [U | U M + Z'], Y = [Y' | V] (Z', Y' the complement terms below).
  * `complement` selects the part of dU, dV outside span(U), span(V):
  *   0 = none (in-subspace terms only),
  *   1 = (I - UU') grad_U D^-1 and (I - VV') grad_V D^-1, exact when rank(A) = k,
- *   2 = exact for any A: two CG solves per triplet on the projected normal
- *       operators (I - UU') A A' (I - UU') and (I - VV') A'A (I - VV'), run
- *       through m's own products (needs D[k-1] > the (k+1)-th singular value).
+ *   2 = exact for any A: one CG system per triplet and side on the projected
+ *       normal operators (I - UU') A A' (I - UU') and (I - VV') A'A (I - VV'),
+ *       all 2k run in lock step through m's own products, to a residual of
+ *       cg_tolerance relative (needs D[k-1] > the (k+1)-th singular value).
  * grad_center (cols, nullable) receives -Y (X' 1), the gradient with respect to
  * a column-centre vector c of A = B - 1 c'.  *cg_iterations (nullable) gets the
- * total CG iterations.  k <= 64. */
+ * number of lock-step CG iterations.  k <= 64. */
 int irlba_hip_svd_backward(const irlba_hip_matrix* m, int64_t k, const double* U, const double* D, const double* V,
                            const double* grad_D, const double* grad_U, const double* grad_V, int complement,
                            int32_t max_cg_iterations, double cg_tolerance, double* X, double* Y, double* grad_center,
@@ -195,11 +197,13 @@ int irlba_hip_svd_backward(const irlba_hip_matrix* m, int64_t k, const double* U
 
 /* out[e] = X[row_e, :] . Y[col_e, :] for every stored entry e of a compressed
  * pattern (indices int32, pointers int64; column_major = CSC), with X nrow x rank
- * and Y ncol x rank column-major: the gradient of the stored values of a sparse
- * matrix from the factored dL/dA of irlba_hip_svd_backward.  rank <= 128. */
+ * and Y ncol x rank column-major FP64: the gradient of the stored values of a
+ * sparse matrix from the factored dL/dA of irlba_hip_svd_backward.  `out_type`
+ * F64 or F32 (the values' type): F32 output rounds the factors to FP32 and
+ * accumulates each dot product in FP64.  rank <= 128. */
 int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_t* indices,
                     const int64_t* pointers, int column_major, int64_t rank, const double* X, const double* Y,
-                    double* out, int device_pointers);
+                    void* out, int out_type, int device_pointers);
 
 #ifdef __cplusplus
 }

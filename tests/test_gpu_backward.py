@@ -111,6 +111,28 @@ def test_sparse_backward_and_sddmm(nr, nc, column_major):
     assert np.abs(gvals - direct).max() <= 1e-13 * np.abs(direct).max()
 
 
+@pytest.mark.parametrize("R", [1, 3, 10, 32, 64, 128])
+@pytest.mark.parametrize("column_major", [True, False])
+def test_sddmm_ranks_and_precisions(R, column_major):
+    # Every lanes-per-entry variant (R up to 128, FP32 and FP64 factors) against numpy.
+    nr, nc = 700, 450
+    vals, idx, ptr = O.gen_nb_sparse(nr if column_major else nc, nc if column_major else nr, 0.04, R, f32=False)
+    rng = np.random.default_rng(R)
+    X = rng.standard_normal((nr, R))
+    Y = rng.standard_normal((nc, R))
+    lines = nc if column_major else nr
+    owner = np.repeat(np.arange(lines), np.diff(ptr))
+    rows, cols = (idx, owner) if column_major else (owner, idx)
+    ref = np.einsum("ij,ij->i", X[rows], Y[cols])
+    out64 = irlba.sddmm(nr, nc, idx, ptr, X, Y, column_major=column_major)
+    assert out64.dtype == np.float64
+    assert np.abs(out64 - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+    out32 = irlba.sddmm(nr, nc, idx, ptr, X, Y, column_major=column_major, out_type="f32")
+    assert out32.dtype == np.float32
+    # FP32 factors: each product carries ~2^-24 relative rounding per factor.
+    assert np.abs(out32 - ref).max() <= 1e-6 * np.sqrt(R) * np.abs(X).max() * np.abs(Y).max() * 4
+
+
 def test_sddmm_rank_bounds_and_bad_pattern():
     X = np.ones((4, 2))
     Y = np.ones((3, 2))
