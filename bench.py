@@ -197,9 +197,77 @@ def backward_leg(args):
     return res
 
 
+def dense_leg(args):
+    """Dense mixed precision: 200000 x 2000 = rank-30 planted signal (s_i = 100 * 0.9^i,
+    random orthonormal factors) + N(0,1) noise, stored as BF16 and as FP32 (FP32
+    accumulation) and as FP64; top-20 triplets (compute, extra_work 7)."""
+    import numpy as np
+    import torch
+
+    import cppirlba_amd as irlba
+
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed)
+    m, n, r, k = args.dense_rows, args.dense_cols, 30, 20
+    t0 = time.perf_counter()
+    Q1, _ = torch.linalg.qr(torch.randn(m, r, generator=gen, device=dev, dtype=torch.float64))
+    Q2, _ = torch.linalg.qr(torch.randn(n, r, generator=gen, device=dev, dtype=torch.float64))
+    s = 100.0 * 0.9 ** torch.arange(r, device=dev, dtype=torch.float64)
+    A64 = torch.randn(m, n, generator=gen, device=dev, dtype=torch.float64)
+    A64.addmm_(Q1 * s, Q2.T)
+    del Q1, Q2
+    torch.cuda.synchronize()
+    log(f"  dense {m}x{n} generated in {time.perf_counter() - t0:.2f} s")
+    init = np.random.default_rng(4321).standard_normal(n)
+    opts = irlba.Options(initial=init)
+    res = {"rows": m, "cols": n, "number": k, "planted_rank": r, "extra_work": opts.extra_work}
+    ctx = irlba.default_context()
+    runs = {}
+    kinds = {"bf16": torch.bfloat16, "f32": torch.float32, "f64": torch.float64}
+    for name in args.dense_storages.split(","):
+        dtype = kinds[name]
+        At = A64.to(dtype)
+        h = irlba.HipDenseMatrix(At, storage=name)
+        del At
+        torch.cuda.empty_cache()
+        times = []
+        for _ in range(max(1, args.dense_reps)):
+            t0 = time.perf_counter()
+            out = irlba.compute(h, k, opts)
+            times.append(time.perf_counter() - t0)
+        st = ctx.stats()
+        fwd_ms = st["forward_kernel_ms"] / max(1, st["forward_products"])
+        adj_ms = st["adjoint_kernel_ms"] / max(1, st["adjoint_products"])
+        runs[name] = out
+        res[name] = {
+            "solve_s": round(float(np.median(times)), 4), "iterations": int(out.iterations),
+            "converged": bool(out.converged),
+            "forward_ms": round(fwd_ms, 4), "adjoint_ms": round(adj_ms, 4),
+            "forward_GBps": round(st["forward_bytes"] / (fwd_ms * 1e6), 1),
+            "adjoint_GBps": round(st["adjoint_bytes"] / (adj_ms * 1e6), 1),
+        }
+        log(f"  dense {name}: {np.median(times):.3f} s, {out.iterations} restarts, fwd {fwd_ms:.3f} ms "
+            f"({res[name]['forward_GBps']} GB/s), adj {adj_ms:.3f} ms ({res[name]['adjoint_GBps']} GB/s)")
+        del h
+        torch.cuda.empty_cache()
+    ref = runs.get("f64")
+    for name in [n for n in runs if n != "f64" and ref is not None]:
+        o = runs[name]
+        res[name]["sv_rel_err_vs_fp64"] = float(np.max(np.abs(o.D - ref.D) / ref.D))
+        qa, _ = np.linalg.qr(o.V)
+        qb, _ = np.linalg.qr(ref.V)
+        cos = np.linalg.svd(qa.T @ qb, compute_uv=False)
+        res[name]["max_subspace_angle_vs_fp64"] = float(np.arccos(np.clip(cos.min(), -1, 1)))
+        res[name]["speedup_vs_fp64"] = round(res["f64"]["solve_s"] / res[name]["solve_s"], 3)
+    del A64
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", choices=["solve", "spmv", "backward"], default="solve")
+    ap.add_argument("--mode", choices=["solve", "spmv", "backward", "dense"], default="solve")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--storage", default="f32")
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +282,11 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-backward", action="store_true")
+    ap.add_argument("--skip-dense", action="store_true")
+    ap.add_argument("--dense-rows", type=int, default=200000)
+    ap.add_argument("--dense-cols", type=int, default=2000)
+    ap.add_argument("--dense-reps", type=int, default=1)
+    ap.add_argument("--dense-storages", default="bf16,f32,f64")
     ap.add_argument("--backward-exact-sparse", action="store_true",
                     help="also time the exact (CG) complement on the sparse backward case")
     ap.add_argument("--cpu-sample-cols", type=int, default=33334)
@@ -228,6 +301,9 @@ def main():
         return
     if args.mode == "backward":
         print(json.dumps(backward_leg(args)), flush=True)
+        return
+    if args.mode == "dense":
+        print(json.dumps(dense_leg(args)), flush=True)
         return
 
     import numpy as np

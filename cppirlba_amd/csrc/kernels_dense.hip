@@ -174,6 +174,24 @@ __global__ void k_dense_reduce(const AT* __restrict__ partial, int splits, i64 n
     }
 }
 
+// First stage for many splits: block (x, g) sums splits [g*per, (g+1)*per) of
+// 256 consecutive outputs into partial2[g][.], in a fixed order.
+template <typename AT>
+__global__ __launch_bounds__(256) void k_dense_fold(const AT* __restrict__ partial, int splits, int per, i64 n_out,
+                                                    AT* __restrict__ partial2) {
+    const i64 m = static_cast<i64>(blockIdx.x) * 256 + threadIdx.x;
+    if (m >= n_out) return;
+    const int s0 = blockIdx.y * per, s1 = min(splits, s0 + per);
+    AT acc[4] = {0, 0, 0, 0};
+    int s = s0;
+    for (; s + 3 < s1; s += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] += partial[static_cast<i64>(s + u) * n_out + m];
+    }
+    for (; s < s1; ++s) acc[0] += partial[static_cast<i64>(s) * n_out + m];
+    partial2[static_cast<i64>(blockIdx.y) * n_out + m] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
 template <typename VT, typename AT>
 void dense_matvec_typed(const DenseDesc& d, bool adjoint, const Prologue& pro, const Epilogue& epi, void* xs_scratch,
                         double* gamma_scratch, void* partial_v, i64 partial_cap, hipStream_t s, hipEvent_t ev0,
@@ -231,9 +249,25 @@ void dense_matvec_typed(const DenseDesc& d, bool adjoint, const Prologue& pro, c
         if (ev1) IRLBA_HIP_CHECK(hipEventRecord(ev1, s));
         splits = static_cast<int>(nchunk);
     }
+    // Many splits over few outputs (a tall row-major A'u): fold them in two
+    // stages so that the serial chain per output stays short.  The second
+    // stage's buffer is the tail of the partial buffer.
+    const AT* red_in = partial;
+    constexpr int kFold = 64;
+    if (splits > 2 * kFold && (static_cast<i64>(splits) + kFold) * n_out <= partial_cap) {
+        const int per = (splits + kFold - 1) / kFold;
+        const int groups = (splits + per - 1) / per;
+        AT* partial2 = partial + static_cast<i64>(splits) * n_out;
+        const dim3 g(static_cast<unsigned>((n_out + 255) / 256), static_cast<unsigned>(groups));
+        hipLaunchKernelGGL(k_dense_fold<AT>, g, dim3(256), 0, s, static_cast<const AT*>(partial), splits, per, n_out,
+                           partial2);
+        IRLBA_HIP_LAUNCH_CHECK();
+        red_in = partial2;
+        splits = groups;
+    }
     const i64 rgrid = std::max<i64>(1, std::min<i64>((n_out + 255) / 256, 8192));
-    hipLaunchKernelGGL(k_dense_reduce<AT>, dim3(static_cast<unsigned>(rgrid)), dim3(256), 0, s,
-                       static_cast<const AT*>(partial), splits, n_out, gamma_scratch, pro_blocks, epi);
+    hipLaunchKernelGGL(k_dense_reduce<AT>, dim3(static_cast<unsigned>(rgrid)), dim3(256), 0, s, red_in, splits, n_out,
+                       gamma_scratch, pro_blocks, epi);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
@@ -328,7 +362,9 @@ i64 dense_partial_elems(const DenseDesc& d) {
     const i64 n_lines = d.col_major ? d.nc : d.nr;
     const i64 Ln = d.col_major ? d.nr : d.nc;
     const i64 dot_bound = n_lines * std::max<i64>(1, (Ln + 8191) / 8192 + 1);
-    const i64 axpy_bound = Ln * std::max<i64>(1, (n_lines + 63) / 64 + 1);
+    // axpy: at most min(2048, n_lines / 64) chunks, plus 64 rows for the second
+    // stage of the split reduction.
+    const i64 axpy_bound = Ln * (std::max<i64>(1, std::min<i64>(2048, (n_lines + 63) / 64)) + 1 + 64);
     return std::max(dot_bound, axpy_bound);
 }
 

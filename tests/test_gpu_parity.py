@@ -105,9 +105,11 @@ def test_generator_bit_identical_to_oracle():
 
 
 # ---------------------------------------------------------------- dense products
-@pytest.mark.parametrize("nr,nc", [(20, 10), (1000, 200), (33, 1001), (4099, 17)])
+@pytest.mark.parametrize("nr,nc", [(20, 10), (1000, 200), (33, 1001), (4099, 17), (40000, 300)])
 @pytest.mark.parametrize("order", ["F", "C"])
 def test_dense_products(nr, nc, order):
+    # (40000, 300) row-major: A'u splits over more than 128 line chunks and
+    # reduces in two stages.
     rng = np.random.default_rng(nr + nc)
     A = np.asarray(rng.standard_normal((nr, nc)), order=order)
     v = rng.standard_normal(nc)
@@ -117,6 +119,7 @@ def test_dense_products(nr, nc, order):
     assert _rel(H.adjoint_multiply(u), A.T @ u) < 1e-13
     H32 = irlba.HipDenseMatrix(A.astype(np.float32), storage="f32")
     assert _rel(H32.multiply(v), A.astype(np.float32).astype(np.float64) @ v) < 2e-6
+    assert _rel(H32.adjoint_multiply(u), A.astype(np.float32).astype(np.float64).T @ u) < 2e-6
     Hb = irlba.HipDenseMatrix(A, storage="bf16")
     assert _rel(Hb.multiply(v), A @ v) < 2e-2
     assert _rel(Hb.adjoint_multiply(u), A.T @ u) < 2e-2
