@@ -428,6 +428,10 @@ def main():
         log("SVD backward (top-16, dense and sparse)")
         out["svd_backward"] = backward_leg(args)
 
+    if not args.skip_dense:
+        log("dense mixed precision (BF16 / FP32 / FP64 storage)")
+        out["dense"] = dense_leg(args)
+
     if not args.skip_cpu and nnz:
         log("CPU baseline (oracle restatement of ParallelSparseMatrix) on a column slice")
         out["cpu_baseline"] = cpu_baseline(nr, nc, args.density, args.seed, nnz, n_fwd, n_adj,

@@ -18,41 +18,60 @@ int cgs_blocks(i64 L) {
 static inline i64 cgs_chunk(i64 L, int G) { return (L + G - 1) / G; }
 
 // ------------------------------------------------------------------ t = M' f (partials)
-// Thread per row (coalesced column reads, one f load per row), columns in
-// batches of 32 held in registers; the 32 per-lane partial sums are reduced
-// across the wave by recursive halving (lanes exchange the half of the batch
-// the partner keeps: 16+8+4+2+1 shuffles, then one xor-add), which leaves
-// column q of the batch in lanes 2q and 2q+1; four waves combine through LDS.
-constexpr int kDotCols = 32;
+// Thread per row (coalesced column reads), U rows per thread in flight, the
+// columns in batches of C held in registers (C * U accumulators).  Each batch's
+// per-lane sums are reduced across the wave by recursive halving (lanes
+// exchange the half of the batch the partner keeps), which leaves column q of
+// the batch spread over 64 / C lanes; a few xor-adds finish it and four waves
+// combine through LDS.  The basis is streamed once per call, so its loads are
+// nontemporal (they would only evict the vectors from the caches).  Long
+// vectors use 8-column batches with 4 rows per thread (few registers, many
+// loads in flight); short ones 32-column batches with one row per thread (few
+// batches, each a single round of loads).
 
-// ~1024-row chunks for long vectors (up to 1024 blocks), 256-row chunks for
-// short ones so that at least a few hundred blocks share the work.
+// Long vectors: 1024-row chunks (U = 4 rows per thread, one pass), at most
+// 1024 blocks.  Short ones: 256-row chunks (U = 1) so that enough blocks share
+// the work.
+constexpr i64 kDotLong = 262144;
 int cgs_dot_blocks(i64 L) {
-    const i64 by_1k = (L + 1023) / 1024, by_256 = (L + 255) / 256;
-    return static_cast<int>(std::max<i64>(1, std::min<i64>(1024, std::max(by_1k, std::min<i64>(by_256, 256)))));
+    const i64 per = L >= kDotLong ? 1024 : 256;
+    return static_cast<int>(std::max<i64>(1, std::min<i64>(1024, (L + per - 1) / per)));
 }
 
+template <int C, int U>
 __global__ __launch_bounds__(256) void k_cgs_dots(const double* __restrict__ M, i64 ld, int n,
                                                   const double* __restrict__ f, i64 L, i64 chunk,
                                                   double* __restrict__ partials) {
-    __shared__ double red[4][kDotCols];
+    static_assert(C == 8 || C == 16 || C == 32, "batch width");
+    constexpr int span = 64 / C;  // lanes holding parts of one column after the halving
+    __shared__ double red[4][C];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const i64 r0 = static_cast<i64>(blockIdx.x) * chunk;
     const i64 r1 = min(L, r0 + chunk);
-    for (int cb = 0; cb < n; cb += kDotCols) {
-        double v[kDotCols];
+    for (int cb = 0; cb < n; cb += C) {
+        double v[C];
 #pragma unroll
-        for (int q = 0; q < kDotCols; ++q) v[q] = 0;
-        const int nb = min(kDotCols, n - cb);
-        for (i64 r = r0 + threadIdx.x; r < r1; r += 256) {
-            const double fr = f[r];
-            const double* Mr = M + r + static_cast<i64>(cb) * ld;
+        for (int q = 0; q < C; ++q) v[q] = 0;
+        const int nb = min(C, n - cb);
+        for (i64 r = r0 + threadIdx.x; r < r1; r += 256 * U) {
+            double fr[U];
 #pragma unroll
-            for (int q = 0; q < kDotCols; ++q)
-                if (q < nb) v[q] += Mr[q * ld] * fr;
+            for (int u = 0; u < U; ++u) fr[u] = r + 256 * u < r1 ? f[r + 256 * u] : 0.0;
+            const double* p = M + r + static_cast<i64>(cb) * ld;
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                if (q < nb) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (r + 256 * u < r1) v[q] += __builtin_nontemporal_load(p + 256 * u) * fr[u];
+                }
+                p += ld;
+            }
         }
+        // Halving over offsets 32, 16, ..., span leaves partial sums of column q
+        // in the `span` lanes with lane / span == q; xor-adds total them.
 #pragma unroll
-        for (int half = kDotCols / 2, o = 32; half >= 1; half >>= 1, o >>= 1) {
+        for (int half = C / 2, o = 32; half >= 1; half >>= 1, o >>= 1) {
             const bool upper = (lane & o) != 0;
 #pragma unroll
             for (int j = 0; j < half; ++j) {
@@ -61,8 +80,9 @@ __global__ __launch_bounds__(256) void k_cgs_dots(const double* __restrict__ M, 
                 v[j] = keep + __shfl_xor(send, o, 64);
             }
         }
-        v[0] += __shfl_xor(v[0], 1, 64);
-        if ((lane & 1) == 0) red[w][lane >> 1] = v[0];
+#pragma unroll
+        for (int o = span / 2; o >= 1; o >>= 1) v[0] += __shfl_xor(v[0], o, 64);
+        if ((lane & (span - 1)) == 0) red[w][lane / span] = v[0];
         __syncthreads();
         if (threadIdx.x < nb)
             partials[static_cast<i64>(blockIdx.x) * n + cb + threadIdx.x] =
@@ -74,7 +94,10 @@ __global__ __launch_bounds__(256) void k_cgs_dots(const double* __restrict__ M, 
 void launch_cgs_dots(const double* M, i64 ld, int n, const double* f, i64 L, double* partials, hipStream_t s) {
     if (n <= 0) return;
     const int G = cgs_dot_blocks(L);
-    hipLaunchKernelGGL(k_cgs_dots, dim3(G), dim3(256), 0, s, M, ld, n, f, L, cgs_chunk(L, G), partials);
+    if (L >= kDotLong)
+        hipLaunchKernelGGL((k_cgs_dots<8, 4>), dim3(G), dim3(256), 0, s, M, ld, n, f, L, cgs_chunk(L, G), partials);
+    else
+        hipLaunchKernelGGL((k_cgs_dots<32, 1>), dim3(G), dim3(256), 0, s, M, ld, n, f, L, cgs_chunk(L, G), partials);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
@@ -97,6 +120,9 @@ void launch_cgs_reduce(const double* partials, i64 L, int n, double* t, hipStrea
 }
 
 // ------------------------------------------------------------------ f -= M t, plus sum of squares
+// Thread per row, U rows in flight per thread, columns 4 at a time with
+// nontemporal loads; t staged in LDS.
+template <int U>
 __global__ __launch_bounds__(256) void k_cgs_update(const double* __restrict__ M, i64 ld, int n,
                                                     const double* __restrict__ t, double* __restrict__ f, i64 L,
                                                     i64 chunk, double* __restrict__ norm_part) {
@@ -107,19 +133,41 @@ __global__ __launch_bounds__(256) void k_cgs_update(const double* __restrict__ M
     const i64 r0 = static_cast<i64>(blockIdx.x) * chunk;
     const i64 r1 = min(L, r0 + chunk);
     double sq = 0;
-    for (i64 r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-        int i = 0;
-        for (; i + 3 < n; i += 4) {
-            a0 += M[r + i * ld] * ts[i];
-            a1 += M[r + (i + 1) * ld] * ts[i + 1];
-            a2 += M[r + (i + 2) * ld] * ts[i + 2];
-            a3 += M[r + (i + 3) * ld] * ts[i + 3];
+    for (i64 r = r0 + threadIdx.x; r < r1; r += 256 * U) {
+        bool in[U];
+        double acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            in[u] = r + 256 * u < r1;
+            acc[u] = 0;
         }
-        for (; i < n; ++i) a0 += M[r + i * ld] * ts[i];
-        const double v = f[r] - ((a0 + a1) + (a2 + a3));
-        f[r] = v;
-        sq += v * v;
+        const double* p = M + r;
+        int i = 0;
+        for (; i + 4 <= n; i += 4) {
+            double m[4][U];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int u = 0; u < U; ++u) m[c][u] = in[u] ? __builtin_nontemporal_load(p + c * ld + 256 * u) : 0.0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int u = 0; u < U; ++u) acc[u] += m[c][u] * ts[i + c];
+            p += 4 * ld;
+        }
+        for (; i < n; ++i) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (in[u]) acc[u] += __builtin_nontemporal_load(p + 256 * u) * ts[i];
+            p += ld;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (in[u]) {
+                const double v = f[r + 256 * u] - acc[u];
+                f[r + 256 * u] = v;
+                sq += v * v;
+            }
     }
     double dummy = 0;
     block_sum2(sq, dummy, red);
@@ -128,10 +176,14 @@ __global__ __launch_bounds__(256) void k_cgs_update(const double* __restrict__ M
 
 void launch_cgs_update(const double* M, i64 ld, int n, const double* t, double* f, i64 L, double* norm_part,
                        hipStream_t s) {
-    const int G = cgs_blocks(L);
+    const int G = cgs_blocks(L);  // the norm finalisation sums exactly these partials
     const size_t lds = sizeof(double) * static_cast<size_t>(std::max(n, 1));
     if (lds > 60000) throw std::runtime_error("working subspace too large for the CGS update kernel");
-    hipLaunchKernelGGL(k_cgs_update, dim3(G), dim3(256), lds, s, M, ld, n, t, f, L, cgs_chunk(L, G), norm_part);
+    const i64 chunk = cgs_chunk(L, G);
+    if (chunk >= 512)
+        hipLaunchKernelGGL(k_cgs_update<4>, dim3(G), dim3(256), lds, s, M, ld, n, t, f, L, chunk, norm_part);
+    else
+        hipLaunchKernelGGL(k_cgs_update<1>, dim3(G), dim3(256), lds, s, M, ld, n, t, f, L, chunk, norm_part);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
@@ -286,7 +338,7 @@ __global__ __launch_bounds__(256) void k_rotate(const double* M, i64 ld, int wor
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const i64 row = row0 + s * 16 + lr;
-            a[s] = (kidx < work && row < L) ? M[row + static_cast<i64>(kidx) * ld] : 0.0;
+            a[s] = (kidx < work && row < L) ? __builtin_nontemporal_load(M + row + static_cast<i64>(kidx) * ld) : 0.0;
         }
     };
     auto load_b = [&](int kidx, double (&b)[NCT]) {
@@ -332,7 +384,8 @@ __global__ __launch_bounds__(256) void k_rotate(const double* M, i64 ld, int wor
 #pragma unroll
         for (int cc = 0; cc < 16; cc += COLS_PER_STORE) {
             const int col = c * 16 + cc + sub;
-            if (col_begin + col < k && row < L) out[row + static_cast<i64>(col_begin + col) * ld_out] = st[(cc + sub) * LDR + rr];
+            if (col_begin + col < k && row < L)
+                __builtin_nontemporal_store(st[(cc + sub) * LDR + rr], out + row + static_cast<i64>(col_begin + col) * ld_out);
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_s_waitcnt(0xc07f);

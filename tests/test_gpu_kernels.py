@@ -12,7 +12,9 @@ pytestmark = pytest.mark.gpu
 irlba = pytest.importorskip("cppirlba_amd")
 
 
-@pytest.mark.parametrize("rows,cols,n", [(5, 4, 4), (1000, 70, 33), (100003, 40, 40), (300, 9, 0)])
+# (300007, 45, 45): long vectors take the 4-rows-per-thread dot and update kernels.
+@pytest.mark.parametrize("rows,cols,n", [(5, 4, 4), (1000, 70, 33), (100003, 40, 40), (300, 9, 0), (300007, 45, 45),
+                                         (262144, 13, 11)])
 def test_orthogonalize_vs_oracle(rows, cols, n):
     rng = np.random.default_rng(rows + n)
     M, _ = np.linalg.qr(rng.standard_normal((rows, cols)))
