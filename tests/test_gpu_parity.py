@@ -317,6 +317,47 @@ def test_pca_sparse_vs_oracle_and_stats():
     assert np.all(np.abs(res.scores.sum(axis=0)) < 1e-8)
 
 
+def test_pca_full_size_fp64_and_mixed():
+    # 30000 x 100000 at 5% (1.5e8 nnz), FP64 storage, pca top-50 with centring and
+    # scaling, extra_work 20: too large for the oracle, so size-independent checks.
+    #  * the bases are orthonormal, the variances sorted;
+    #  * scores = (A - 1c') D^-1 rotation through an independent product route
+    #    (decorator handles built from the column statistics), to 1e-9;
+    #  * the adjoint residual |A_c' u_i - s_i v_i| <= 1e-5 s_1 (the convergence test);
+    #  * every column of the centred, scaled matrix has variance 1, so the variances
+    #    are bounded by their total cols and each lies in (0, cols];
+    #  * the mixed-precision run (FP32 values and products) gives singular values
+    #    within 1e-5 relative of the FP64 run.
+    nr, nc, k = 30000, 100000, 50
+    A = irlba.generate_sparse(nr, nc, 0.05, 17, storage="f64", accumulate="f64")
+    init = np.random.default_rng(5).standard_normal(nc)
+    opts = irlba.Options(extra_work=20, initial=init)
+    res = irlba.pca(A, True, True, k, opts)
+    assert res.converged
+    var = res.variances
+    assert np.all(np.diff(var) <= 0) and var[-1] > 0 and var.sum() <= nc
+    R = res.rotation
+    assert np.abs(R.T @ R - np.eye(k)).max() < 1e-10
+    c, s = irlba.column_stats(A)
+    Ac = irlba.ScaledMatrix(irlba.CenteredMatrix(A, c), s, column=True, divide=True)
+    sv = np.sqrt(var * (nr - 1))
+    U = res.scores / sv[None, :]
+    assert np.abs(U.T @ U - np.eye(k)).max() < 1e-10
+    worst_fwd, worst_adj = 0.0, 0.0
+    for i in range(0, k, 7):
+        y = Ac.multiply(R[:, i])
+        worst_fwd = max(worst_fwd, np.linalg.norm(y - res.scores[:, i]) / sv[i])
+        z = Ac.adjoint_multiply(U[:, i])
+        worst_adj = max(worst_adj, np.linalg.norm(z - sv[i] * R[:, i]) / sv[0])
+    assert worst_fwd < 1e-9
+    assert worst_adj < 1e-5
+    del A, Ac
+    M = irlba.generate_sparse(nr, nc, 0.05, 17, storage="f32", accumulate="f32")
+    mixed = irlba.pca(M, True, True, k, opts)
+    assert mixed.converged
+    assert np.max(np.abs(np.sqrt(mixed.variances) - np.sqrt(var)) / np.sqrt(var)) < 1e-5
+
+
 def test_pca_failures_and_zero_variance():
     with pytest.raises(irlba.HipError, match="cannot scale"):
         irlba.pca(O.create_random_matrix(1, 10), True, True, 1)
