@@ -831,9 +831,10 @@ struct Engine {
         launch_normalize(f, L, norm_part.get(), cgs_blocks(L), norm_dev, dst, s);
     }
 
-    // lanczos.hpp:73-86
-    void first(i64 start) {
-        op.apply(false, Vc(start), Wn.get(), nullptr, nullptr);
+    // lanczos.hpp:73-86.  `product_ready`: W_next already holds A V[:, start]
+    // (queued by the restart before the host SVD, see run_compute).
+    void first(i64 start, bool product_ready = false) {
+        if (!product_ready) op.apply(false, Vc(start), Wn.get(), nullptr, nullptr);
         cgs(W.get(), nr, start, Wn.get());
         norm_to(Wn.get(), nr, dS(start), Wc(start));
     }
@@ -876,8 +877,8 @@ struct Engine {
     }
 
     // run_lanczos_bidiagonalization (lanczos.hpp:54-143) from column `start`; fills B.
-    void sweep(i64 start, HostMat& B) {
-        first(start);
+    void sweep(i64 start, HostMat& B, bool product_ready = false) {
+        first(start, product_ready);
         launch_from(2 * start);
         read_norms();
         if (S[start] < eps) throw std::runtime_error("starting vector near the null space of the input matrix");
@@ -1009,7 +1010,15 @@ std::pair<bool, int> run_compute(irlba_hip_context* ctx, Op& op, i64 number, con
     const double svtol = o.singular_value_ratio_tolerance >= 0 ? o.singular_value_ratio_tolerance : tol;
 
     for (; iter < o.max_iterations; ++iter) {
-        eg.sweep(k, B);
+        eg.sweep(k, B, iter > 0);
+        // The next sweep starts from V[:, k] = F / R_F (compute.hpp:280) whatever
+        // k the restart picks, so its first product A V[:, k] is queued now and
+        // runs on the GPU while the host computes the SVD of B.  F is scaled in
+        // place and copied into V[:, k] after the V rotation has read the old basis.
+        // (Wasted once, when this restart converges.)
+        const double R_F = eg.R[work - 1];
+        launch_scale_copy(eg.F.get(), nc, R_F, eg.F.get(), s);
+        eg.op.apply(false, eg.F.get(), eg.Wn.get(), nullptr, nullptr);
         const auto ts = std::chrono::steady_clock::now();
         host_svd(B, BU, BS, BV);
         st.host_svd_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
@@ -1018,7 +1027,6 @@ std::pair<bool, int> run_compute(irlba_hip_context* ctx, Op& op, i64 number, con
             converged = true;
             break;
         }
-        const double R_F = eg.R[work - 1];
         for (i64 j = 0; j < work; ++j) res[j] = R_F * BU(work - 1, j);
         i64 n_conv = 0;
         if (iter > 0) {
@@ -1038,7 +1046,7 @@ std::pair<bool, int> run_compute(irlba_hip_context* ctx, Op& op, i64 number, con
         prevS = BS;
         k = update_k(k, requested, n_conv, work);
         eg.rotate(0, eg.V.get(), nc, BV, k, eg.V.get(), nc);
-        launch_scale_copy(eg.F.get(), nc, R_F, eg.Vc(k), s);
+        IRLBA_HIP_CHECK(hipMemcpyAsync(eg.Vc(k), eg.F.get(), sizeof(double) * nc, hipMemcpyDeviceToDevice, s));
         eg.rotate(1, eg.W.get(), nr, BU, k, eg.W.get(), nr);
         std::fill(B.a.begin(), B.a.end(), 0.0);
         for (i64 l = 0; l < k; ++l) {

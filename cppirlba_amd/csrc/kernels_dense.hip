@@ -161,10 +161,27 @@ __global__ __launch_bounds__(256) void k_axpy_lines(const VT* __restrict__ A, i6
 template <typename AT>
 __global__ void k_dense_reduce(const AT* __restrict__ partial, int splits, i64 n_out, const double* __restrict__ gpart,
                                int n_gpart, Epilogue epi) {
+    // The rank-1 dot products: the first wave of each block sums the prologue's
+    // partials (in a fixed order) and shares them through LDS.
+    __shared__ double gsh[2];
     double gamma[2] = {0, 0};
-    for (int b = 0; b < n_gpart; ++b) {
-        gamma[0] += gpart[2 * b];
-        gamma[1] += gpart[2 * b + 1];
+    if (epi.n_terms > 0) {
+        if (threadIdx.x < kWave) {
+            double g0 = 0, g1 = 0;
+            for (int b = threadIdx.x; b < n_gpart; b += kWave) {
+                g0 += gpart[2 * b];
+                g1 += gpart[2 * b + 1];
+            }
+            g0 = wave_sum(g0);
+            g1 = wave_sum(g1);
+            if (threadIdx.x == 0) {
+                gsh[0] = g0;
+                gsh[1] = g1;
+            }
+        }
+        __syncthreads();
+        gamma[0] = gsh[0];
+        gamma[1] = gsh[1];
     }
     for (i64 m = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; m < n_out;
          m += static_cast<i64>(gridDim.x) * blockDim.x) {

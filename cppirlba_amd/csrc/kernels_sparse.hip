@@ -444,10 +444,26 @@ __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
 template <typename AT>
 __global__ void k_spmv_reduce(const AT* __restrict__ partial, int n_tiles, i64 n_major,
                               const double* __restrict__ gamma_part, Epilogue epi) {
+    // The rank-1 dot products: one wave sums the per-tile parts, shared through LDS.
+    __shared__ double gsh[2];
     double gamma[2] = {0, 0};
-    for (int t = 0; t < n_tiles; ++t) {
-        gamma[0] += gamma_part[2 * t];
-        gamma[1] += gamma_part[2 * t + 1];
+    if (epi.n_terms > 0) {
+        if (threadIdx.x < kWave) {
+            double g0 = 0, g1 = 0;
+            for (int t = threadIdx.x; t < n_tiles; t += kWave) {
+                g0 += gamma_part[2 * t];
+                g1 += gamma_part[2 * t + 1];
+            }
+            g0 = wave_sum(g0);
+            g1 = wave_sum(g1);
+            if (threadIdx.x == 0) {
+                gsh[0] = g0;
+                gsh[1] = g1;
+            }
+        }
+        __syncthreads();
+        gamma[0] = gsh[0];
+        gamma[1] = gsh[1];
     }
     for (i64 m = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; m < n_major;
          m += static_cast<i64>(gridDim.x) * blockDim.x) {
