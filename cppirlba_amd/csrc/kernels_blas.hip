@@ -48,7 +48,10 @@ __global__ __launch_bounds__(256) void k_cgs_dots(const double* __restrict__ M, 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const i64 r0 = static_cast<i64>(blockIdx.x) * chunk;
     const i64 r1 = min(L, r0 + chunk);
-    for (int cb = 0; cb < n; cb += C) {
+    // Column batches are spread over gridDim.y: with one batch per block (long
+    // vectors) the blocks resident on a CU touch only C columns, so the pages
+    // of the basis they walk stay within the CU's translation cache.
+    for (int cb = blockIdx.y * C; cb < n; cb += C * gridDim.y) {
         double v[C];
 #pragma unroll
         for (int q = 0; q < C; ++q) v[q] = 0;
@@ -95,7 +98,8 @@ void launch_cgs_dots(const double* M, i64 ld, int n, const double* f, i64 L, dou
     if (n <= 0) return;
     const int G = cgs_dot_blocks(L);
     if (L >= kDotLong)
-        hipLaunchKernelGGL((k_cgs_dots<8, 4>), dim3(G), dim3(256), 0, s, M, ld, n, f, L, cgs_chunk(L, G), partials);
+        hipLaunchKernelGGL((k_cgs_dots<8, 4>), dim3(G, (n + 7) / 8), dim3(256), 0, s, M, ld, n, f, L, cgs_chunk(L, G),
+                           partials);
     else
         hipLaunchKernelGGL((k_cgs_dots<32, 1>), dim3(G), dim3(256), 0, s, M, ld, n, f, L, cgs_chunk(L, G), partials);
     IRLBA_HIP_LAUNCH_CHECK();
