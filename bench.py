@@ -396,6 +396,10 @@ def main():
         key = f"{dominant}_hbm_bytes_per_launch"
         if key in tr:
             out["roofline"]["traffic"] = tr[key]
+            # Measured HBM bytes over the same launch time: the stored layout keeps
+            # 2-byte in-tile indices, so it moves less than the algorithmic figure.
+            out["roofline"]["traffic_GBps"] = round(tr[key] / (dom_ms * 1e6), 1)
+            out["roofline"]["traffic_frac"] = round(tr[key] / (dom_ms * 1e6) / HBM_PEAK_GBS, 4)
 
     if not args.skip_fp64:
         del mixed

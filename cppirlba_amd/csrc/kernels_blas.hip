@@ -24,10 +24,10 @@ static inline i64 cgs_chunk(i64 L, int G) { return (L + G - 1) / G; }
 // exchange the half of the batch the partner keeps), which leaves column q of
 // the batch spread over 64 / C lanes; a few xor-adds finish it and four waves
 // combine through LDS.  The basis is streamed once per call, so its loads are
-// nontemporal (they would only evict the vectors from the caches).  Long
-// vectors use 8-column batches with 4 rows per thread (few registers, many
-// loads in flight); short ones 32-column batches with one row per thread (few
-// batches, each a single round of loads).
+// nontemporal (they would only evict the vectors from the caches).  8-column
+// batches, one per block (grid.y), with 4 rows per thread for long vectors and
+// one for short ones: few registers, many loads in flight, and enough blocks
+// for short vectors too.
 
 // Long vectors: 1024-row chunks (U = 4 rows per thread, one pass), at most
 // 1024 blocks.  Short ones: 256-row chunks (U = 1) so that enough blocks share
@@ -101,7 +101,8 @@ void launch_cgs_dots(const double* M, i64 ld, int n, const double* f, i64 L, dou
         hipLaunchKernelGGL((k_cgs_dots<8, 4>), dim3(G, (n + 7) / 8), dim3(256), 0, s, M, ld, n, f, L, cgs_chunk(L, G),
                            partials);
     else
-        hipLaunchKernelGGL((k_cgs_dots<32, 1>), dim3(G), dim3(256), 0, s, M, ld, n, f, L, cgs_chunk(L, G), partials);
+        hipLaunchKernelGGL((k_cgs_dots<8, 1>), dim3(G, (n + 7) / 8), dim3(256), 0, s, M, ld, n, f, L, cgs_chunk(L, G),
+                           partials);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
@@ -124,9 +125,9 @@ void launch_cgs_reduce(const double* partials, i64 L, int n, double* t, hipStrea
 }
 
 // ------------------------------------------------------------------ f -= M t, plus sum of squares
-// Thread per row, U rows in flight per thread, columns 4 at a time with
-// nontemporal loads; t staged in LDS.
-template <int U>
+// Thread per row, U rows in flight per thread, columns CB at a time with
+// nontemporal loads (U * CB = 16 loads in flight per thread); t staged in LDS.
+template <int U, int CB>
 __global__ __launch_bounds__(256) void k_cgs_update(const double* __restrict__ M, i64 ld, int n,
                                                     const double* __restrict__ t, double* __restrict__ f, i64 L,
                                                     i64 chunk, double* __restrict__ norm_part) {
@@ -147,17 +148,17 @@ __global__ __launch_bounds__(256) void k_cgs_update(const double* __restrict__ M
         }
         const double* p = M + r;
         int i = 0;
-        for (; i + 4 <= n; i += 4) {
-            double m[4][U];
+        for (; i + CB <= n; i += CB) {
+            double m[CB][U];
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
+            for (int c = 0; c < CB; ++c)
 #pragma unroll
                 for (int u = 0; u < U; ++u) m[c][u] = in[u] ? __builtin_nontemporal_load(p + c * ld + 256 * u) : 0.0;
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
+            for (int c = 0; c < CB; ++c)
 #pragma unroll
                 for (int u = 0; u < U; ++u) acc[u] += m[c][u] * ts[i + c];
-            p += 4 * ld;
+            p += CB * ld;
         }
         for (; i < n; ++i) {
 #pragma unroll
@@ -185,9 +186,9 @@ void launch_cgs_update(const double* M, i64 ld, int n, const double* t, double* 
     if (lds > 60000) throw std::runtime_error("working subspace too large for the CGS update kernel");
     const i64 chunk = cgs_chunk(L, G);
     if (chunk >= 512)
-        hipLaunchKernelGGL(k_cgs_update<4>, dim3(G), dim3(256), lds, s, M, ld, n, t, f, L, chunk, norm_part);
+        hipLaunchKernelGGL((k_cgs_update<4, 4>), dim3(G), dim3(256), lds, s, M, ld, n, t, f, L, chunk, norm_part);
     else
-        hipLaunchKernelGGL(k_cgs_update<1>, dim3(G), dim3(256), lds, s, M, ld, n, t, f, L, chunk, norm_part);
+        hipLaunchKernelGGL((k_cgs_update<1, 16>), dim3(G), dim3(256), lds, s, M, ld, n, t, f, L, chunk, norm_part);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
