@@ -105,6 +105,13 @@ struct OMat {
     virtual void multiply(const double* in, double* out) const = 0;          // out (rows) = A in
     virtual void adjoint_multiply(const double* in, double* out) const = 0;  // out (cols) = A' in
     virtual void realize(Mat& out) const = 0;
+    // matrix.col(i) of pca.hpp:88 (nr values).  The generic form multiplies by
+    // the i-th unit vector, which is exact; storage types override it with a copy.
+    virtual void column(Index i, double* out) const {
+        std::vector<double> e(cols(), 0.0);
+        e[i] = 1;
+        multiply(e.data(), out);
+    }
 };
 
 // Matrix/simple.hpp:42-44 (A v), :70-72 (A' u), :100-107 (realize).
@@ -125,6 +132,7 @@ struct ODense : OMat {
         for (Index c = 0; c < A.nc; ++c) out[c] = dot(A.col(c), in, A.nr);
     }
     void realize(Mat& out) const override { out = A; }
+    void column(Index i, double* out) const override { std::copy(A.col(i), A.col(i) + A.nr, out); }
 };
 
 // std::thread fork/join, the shape of tests/src/custom_parallel.h:7-19 used as
@@ -270,6 +278,11 @@ struct OSparse : OMat {
                 if (column_major) out(idx[s], p) = x[s]; else out(p, idx[s]) = x[s];
             }
         }
+    }
+    void column(Index i, double* out) const override {
+        if (!column_major) return OMat::column(i, out);
+        std::fill(out, out + secondary, 0.0);
+        for (std::int64_t s = ptr[i]; s < ptr[i + 1]; ++s) out[idx[s]] = x[s];
     }
 };
 
@@ -681,11 +694,9 @@ static void column_stats(const OMat& mat, bool center, bool scale, std::vector<d
         s.assign(nc, 0);
     }
     if (!(center || scale)) return;
-    std::vector<double> e(nc, 0.0), buffer(nr);
+    std::vector<double> buffer(nr);
     for (Index i = 0; i < nc; ++i) {
-        e[i] = 1;
-        mat.multiply(e.data(), buffer.data());  // column i, exact (one nonzero weight of 1)
-        e[i] = 0;
+        mat.column(i, buffer.data());
         double mean = 0;
         if (center) {
             double sum = 0;
