@@ -196,6 +196,17 @@ class _Handle:
     def adjoint_multiply(self, right):
         return self.multiply(right, adjoint=True)
 
+    def multiply_block(self, right, adjoint=False):
+        """M X (or M' X) for the columns of X at once (irlba_hip_multiply_block)."""
+        nr, nc = self.shape
+        X = np.asfortranarray(np.asarray(right, dtype=np.float64))
+        if X.ndim != 2 or X.shape[0] != (nr if adjoint else nc):
+            raise ValueError("block rows do not match the matrix")
+        k = X.shape[1]
+        out = np.zeros((nc if adjoint else nr, k), order="F")
+        check(lib().irlba_hip_multiply_block(self.handle, int(adjoint), k, X.ctypes.data, out.ctypes.data, 0))
+        return out
+
     def realize(self):
         """RealizeWorkspace::realize_copy (interface.hpp:125-130)."""
         nr, nc = self.shape

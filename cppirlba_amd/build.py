@@ -29,6 +29,7 @@ SOURCES = [
     "kernels_blas.hip",
     "kernels_generate.hip",
     "kernels_backward.hip",
+    "kernels_spmm.hip",
 ]
 
 COMMON = [

@@ -129,14 +129,21 @@ struct irlba_hip_context {
     // buffer serves them all.
     std::shared_ptr<void> ws;
     size_t ws_bytes = 0;
-    char* workspace(size_t bytes) {
-        if (bytes > ws_bytes) {
+    char* workspace(size_t bytes) { return grow(ws, ws_bytes, bytes); }
+    // A second one for the multi-vector products, which run inside calls that
+    // hold the first.
+    std::shared_ptr<void> ws_mv;
+    size_t ws_mv_bytes = 0;
+    char* mv_workspace(size_t bytes) { return grow(ws_mv, ws_mv_bytes, bytes); }
+
+    char* grow(std::shared_ptr<void>& buf, size_t& have, size_t bytes) {
+        if (bytes > have) {
             IRLBA_HIP_CHECK(hipStreamSynchronize(stream));
-            ws.reset();
-            ws = dalloc_bytes(bytes);
-            ws_bytes = bytes;
+            buf.reset();
+            buf = dalloc_bytes(bytes);
+            have = bytes;
         }
-        return static_cast<char*>(ws.get());
+        return static_cast<char*>(buf.get());
     }
 };
 
@@ -283,6 +290,63 @@ void apply_handle(const irlba_hip_matrix& m, bool adjoint, const double* x, doub
     }
     if (adjoint) ++ctx->stats.adjoint_products; else ++ctx->stats.forward_products;
     m.base->product(adjoint, pro, epi, e0, e1);
+}
+
+// Y[:, :k] = M X[:, :k] (or M' X) for k column vectors (column-major, leading
+// dimensions ldx, ldy).  Sparse handles stream the matrix once for all k
+// (kernels_spmm.hip); dense ones run k single products.
+void apply_handle_multi(const irlba_hip_matrix& m, bool adjoint, const double* X, i64 ldx, double* Y, i64 ldy,
+                        int k) {
+    auto* sb = dynamic_cast<SparseBase*>(m.base.get());
+    if (!sb || k == 1) {
+        for (int v = 0; v < k; ++v) apply_handle(m, adjoint, X + v * ldx, Y + v * ldy, nullptr, nullptr);
+        return;
+    }
+    irlba_hip_context* ctx = m.ctx;
+    hipStream_t s = ctx->stream;
+    const Affine& a = m.aff;
+    const TiledLines& t = adjoint ? sb->by_col : sb->by_row;
+    const bool x_f32 = t.st == ST_F32 && !sb->acc64;
+    const int kp = spmm_padded_k(k, x_f32);
+    const i64 L_in = adjoint ? sb->nr : sb->nc, L_out = adjoint ? sb->nc : sb->nr;
+    const double* xscale = adjoint ? a.dr.get() : a.dc.get();
+    const double* oscale = adjoint ? a.dc.get() : a.dr.get();
+    const double* tdot[kMaxTerms] = {nullptr, nullptr};
+    const double* tvec[kMaxTerms] = {nullptr, nullptr};
+    bool need_ones = false;
+    for (int j = 0; j < a.n_terms; ++j) {
+        tdot[j] = adjoint ? a.a[j].get() : a.b[j].get();
+        tvec[j] = adjoint ? a.b[j].get() : a.a[j].get();
+        need_ones |= tdot[j] == nullptr;
+    }
+    auto plan = [&](Carve& c, std::vector<void*>& p) {
+        p.clear();
+        p.push_back(c.take<char>(static_cast<i64>(x_f32 ? 4 : 8) * kp * L_in));       // packed X
+        p.push_back(c.take<double>(static_cast<i64>(kp) * L_out));                     // packed Y
+        p.push_back(c.take<double>(2 * k));                                            // gammas
+        p.push_back(c.take<double>(static_cast<i64>(cgs_dot_blocks(L_in)) * k));      // dot partials
+        p.push_back(need_ones ? c.take<double>(L_in) : nullptr);                       // ones
+    };
+    std::vector<void*> wp;
+    Carve sizing;
+    plan(sizing, wp);
+    Carve carve;
+    carve.base = ctx->mv_workspace(sizing.off);
+    plan(carve, wp);
+    double* gam = static_cast<double*>(wp[2]);
+    double* partials = static_cast<double*>(wp[3]);
+    double* ones = static_cast<double*>(wp[4]);
+    if (need_ones) launch_fill(ones, 1.0, L_in, s);
+    for (int j = 0; j < a.n_terms; ++j) {
+        // gamma_j[v] = tdot_j . X[:, v] on the unscaled input (the single product's prologue).
+        launch_cgs_dots(X, ldx, k, tdot[j] ? tdot[j] : ones, L_in, partials, s);
+        launch_cgs_reduce(partials, L_in, k, gam + j * k, s);
+    }
+    launch_spmm_stage(X, ldx, L_in, k, kp, xscale, x_f32, wp[0], s);
+    launch_spmm_lines(t, wp[0], k, x_f32, static_cast<double*>(wp[1]), s);
+    launch_spmm_finish(static_cast<const double*>(wp[1]), L_out, k, kp, oscale, a.n_terms, tvec[0], tvec[1], gam, Y,
+                       ldy, s);
+    if (adjoint) ctx->stats.adjoint_products += k; else ctx->stats.forward_products += k;
 }
 
 // ------------------------------------------------------------------ sparse construction
@@ -1172,12 +1236,8 @@ struct LockstepCg {
 
     // out = sig2 p - P B p for every column, into set.ap.
     void op(CgSet& c, const double* in) {
-        for (int i = 0; i < k; ++i) {
-            apply_handle(*m, c.left, in + static_cast<i64>(i) * c.L, c.tmp + static_cast<i64>(i) * c.Lo, nullptr,
-                         nullptr);
-            apply_handle(*m, !c.left, c.tmp + static_cast<i64>(i) * c.Lo, c.ap + static_cast<i64>(i) * c.L, nullptr,
-                         nullptr);
-        }
+        apply_handle_multi(*m, c.left, in, c.L, c.tmp, c.Lo, k);      // A' in (left) or A in (right)
+        apply_handle_multi(*m, !c.left, c.tmp, c.Lo, c.ap, c.L, k);   // then A (A' in) or A' (A in)
         // ap = basis (basis' ap) - ap + sig2 in
         launch_gram(c.basis, c.L, c.ap, c.L, k, c.L, c.partials, c.proj, ldy, s);
         launch_rotate(c.basis, c.L, k, c.proj, ldy, k, c.uc, c.L, c.L, s);
@@ -1368,7 +1428,7 @@ int run_svd_backward(const irlba_hip_matrix* m, i64 k, const double* U, const do
         auto converged = [&]() {
             IRLBA_HIP_CHECK(hipMemcpyAsync(rr.data(), set[0].rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
             IRLBA_HIP_CHECK(hipMemcpyAsync(rr.data() + k, set[1].rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-            IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+            ctx->sync();  // also returns the products' timing events to the pool
             for (i64 i = 0; i < 2 * k; ++i)
                 if (!(rr[i] <= cg_tol * cg_tol * bb[i])) return false;
             return true;
@@ -1382,10 +1442,8 @@ int run_svd_backward(const irlba_hip_matrix* m, i64 k, const double* U, const do
         cg.finish(set[1]);
         // Z' = Z S + A T into X1;  Y' = A' Z + T S into Y0.  The right set's
         // scratch (rows x k) receives A T.
-        for (i64 i = 0; i < k; ++i) {
-            apply_handle(*m, false, set[1].x + i * nc, set[1].tmp + i * nr, nullptr, nullptr);
-            apply_handle(*m, true, set[0].x + i * nr, Y0 + i * nc, nullptr, nullptr);
-        }
+        apply_handle_multi(*m, false, set[1].x, nc, set[1].tmp, nr, ki);
+        apply_handle_multi(*m, true, set[0].x, nr, Y0, nc, ki);
         launch_cols_axpby(X1, nr, nullptr, 1.0, set[1].tmp, nr, nullptr, 1.0, ki, nr, s);
         launch_cg_coefs(dsig2, nullptr, set[0].alpha, ki, 2, s);  // alpha = sqrt(sig2) = s
         launch_cols_axpby(X1, nr, nullptr, 1.0, set[0].x, nr, set[0].alpha, 1.0, ki, nr, s);
@@ -1610,6 +1668,33 @@ int irlba_hip_multiply(const irlba_hip_matrix* m, int adjoint, const double* rig
         IRLBA_HIP_CHECK(hipMemcpyAsync(din.get(), right, sizeof(double) * nin, hipMemcpyHostToDevice, ctx->stream));
         apply_handle(*m, adjoint != 0, din.get(), dout.get(), nullptr, nullptr);
         IRLBA_HIP_CHECK(hipMemcpyAsync(output, dout.get(), sizeof(double) * nout, hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+    });
+}
+
+int irlba_hip_multiply_block(const irlba_hip_matrix* m, int adjoint, int64_t k, const double* right, double* output,
+                             int device_pointers) {
+    return guarded([&] {
+        require(m && right && output, "null argument");
+        require(k >= 1 && k <= 128, "the block must have between 1 and 128 columns");
+        irlba_hip_context* ctx = m->ctx;
+        DeviceGuard g(ctx->device);
+        const i64 nin = adjoint ? m->base->nr : m->base->nc, nout = adjoint ? m->base->nc : m->base->nr;
+        const double* X = right;
+        double* Y = output;
+        std::shared_ptr<double> din, dout;
+        if (!device_pointers) {
+            din = dalloc<double>(nin * k);
+            dout = dalloc<double>(nout * k);
+            IRLBA_HIP_CHECK(
+                hipMemcpyAsync(din.get(), right, sizeof(double) * nin * k, hipMemcpyHostToDevice, ctx->stream));
+            X = din.get();
+            Y = dout.get();
+        }
+        apply_handle_multi(*m, adjoint != 0, X, nin, Y, nout, static_cast<int>(k));
+        if (!device_pointers)
+            IRLBA_HIP_CHECK(
+                hipMemcpyAsync(output, Y, sizeof(double) * nout * k, hipMemcpyDeviceToHost, ctx->stream));
         ctx->sync();
     });
 }

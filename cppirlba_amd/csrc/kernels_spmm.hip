@@ -1,0 +1,172 @@
+// Multi-vector products Y = M X (k right-hand sides) on the tiled sparse layout,
+// for the lock-step CG of the SVD backward (and any block method).
+//
+// The single-vector product stages its input tile in LDS; with k vectors a tile
+// of all k would not fit, so this kernel keeps the k inputs packed row-major in
+// HBM (row r = k contiguous values, padded to kp) and lets the gathers hit L2 /
+// the MALL: for the backward's k = 16 the packed block is a few MB.  The matrix
+// is streamed once for all k.  One wavefront per output line walks the line's
+// segments in every tile; LPE lanes take one stored entry (each lane a 16-byte
+// piece of the k values), so a wavefront has 64 / LPE entries in flight, and
+// the partial sums of the entry groups are combined by shuffles at the end of
+// the line.  Accumulation is FP64 whatever the storage.
+//
+// The decorators of the handle (M = D_r A D_c - sum_j a_j b_j') are applied by
+// the staging kernel (input scaling, packing) and the finishing kernel (output
+// scaling, rank-1 terms with gamma_j = b_j' X computed beforehand).
+#include "common.hpp"
+
+namespace irlba_hip {
+
+namespace {
+
+template <typename FT>
+struct Piece16;
+template <>
+struct Piece16<float> {
+    typedef f32x4 type;
+    static constexpr int n = 4;
+};
+template <>
+struct Piece16<double> {
+    typedef f64x2 type;
+    static constexpr int n = 2;
+};
+
+template <typename VT, typename FT, int LPE>
+__global__ __launch_bounds__(256) void k_spmm_lines(const i64* __restrict__ ptr, const std::uint16_t* __restrict__ idx,
+                                                    const VT* __restrict__ val, i64 n_major, int n_tiles,
+                                                    int tile_len, const FT* __restrict__ X, int kp,
+                                                    double* __restrict__ Y) {
+    typedef typename Piece16<FT>::type PT;
+    constexpr int VN = Piece16<FT>::n;
+    constexpr int G = kWave / LPE;
+    const int lane = threadIdx.x & 63, q = lane % LPE, g = lane / LPE;
+    const i64 wave = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const i64 nwaves = (static_cast<i64>(gridDim.x) * blockDim.x) >> 6;
+    for (i64 m = wave; m < n_major; m += nwaves) {
+        double acc[VN];
+#pragma unroll
+        for (int v = 0; v < VN; ++v) acc[v] = 0;
+        for (int t = 0; t < n_tiles; ++t) {
+            const i64 seg = static_cast<i64>(t) * n_major + m;
+            const i64 s0 = ptr[seg], s1 = ptr[seg + 1];
+            const i64 base = static_cast<i64>(t) * tile_len;
+            for (i64 s = s0 + g; s < s1; s += G) {
+                const double a = to_double_any(__builtin_nontemporal_load(val + s));
+                if (a == 0) continue;  // padding slots (and stored zeros) contribute nothing
+                const i64 r = base + idx[s];
+                const PT x = *reinterpret_cast<const PT*>(X + r * kp + q * VN);
+#pragma unroll
+                for (int v = 0; v < VN; ++v) acc[v] += a * static_cast<double>(x[v]);
+            }
+        }
+#pragma unroll
+        for (int o = LPE; o < kWave; o <<= 1)
+#pragma unroll
+            for (int v = 0; v < VN; ++v) acc[v] += __shfl_xor(acc[v], o, kWave);
+        if (g == 0) {
+#pragma unroll
+            for (int v = 0; v < VN; ++v) Y[m * kp + q * VN + v] = acc[v];
+        }
+    }
+}
+
+template <typename VT, typename FT, int LPE>
+void spmm_launch(const TiledLines& t, const void* X, int kp, double* Y, hipStream_t s) {
+    const i64 grid = std::max<i64>(1, std::min<i64>((t.n_major + 3) / 4, 1 << 20));
+    hipLaunchKernelGGL((k_spmm_lines<VT, FT, LPE>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, t.ptr, t.idx,
+                       static_cast<const VT*>(t.val), t.n_major, t.n_tiles, t.tile_len, static_cast<const FT*>(X), kp,
+                       Y);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+template <typename VT, typename FT>
+void spmm_dispatch(const TiledLines& t, const void* X, int kp, int lpe, double* Y, hipStream_t s) {
+    switch (lpe) {
+        case 1: spmm_launch<VT, FT, 1>(t, X, kp, Y, s); break;
+        case 2: spmm_launch<VT, FT, 2>(t, X, kp, Y, s); break;
+        case 4: spmm_launch<VT, FT, 4>(t, X, kp, Y, s); break;
+        case 8: spmm_launch<VT, FT, 8>(t, X, kp, Y, s); break;
+        case 16: spmm_launch<VT, FT, 16>(t, X, kp, Y, s); break;
+        case 32: spmm_launch<VT, FT, 32>(t, X, kp, Y, s); break;
+        default: spmm_launch<VT, FT, 64>(t, X, kp, Y, s); break;
+    }
+}
+
+// X_rm[r * kp + v] = xscale[r] * X[r + v * ld] (0 for k <= v < kp).
+template <typename FT>
+__global__ void k_spmm_stage(const double* __restrict__ X, i64 ld, i64 L, int k, int kp,
+                             const double* __restrict__ xscale, FT* __restrict__ out) {
+    const i64 n = L * kp;
+    for (i64 e = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<i64>(gridDim.x) * blockDim.x) {
+        const i64 r = e / kp;
+        const int v = static_cast<int>(e - r * kp);
+        double x = v < k ? X[r + static_cast<i64>(v) * ld] : 0.0;
+        if (xscale && v < k) x *= xscale[r];
+        out[e] = static_cast<FT>(x);
+    }
+}
+
+// Y[m + v * ldy] = oscale[m] * Y_rm[m * kp + v] - sum_j tvec_j[m] * gam[j * k + v].
+__global__ void k_spmm_finish(const double* __restrict__ Yrm, i64 n, int k, int kp, const double* __restrict__ oscale,
+                              int n_terms, const double* __restrict__ t0, const double* __restrict__ t1,
+                              const double* __restrict__ gam, double* __restrict__ Y, i64 ldy) {
+    const i64 total = n * k;
+    for (i64 e = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += static_cast<i64>(gridDim.x) * blockDim.x) {
+        const int v = static_cast<int>(e / n);
+        const i64 m = e - static_cast<i64>(v) * n;
+        double o = Yrm[m * kp + v];
+        if (oscale) o *= oscale[m];
+        if (n_terms > 0) o -= (t0 ? t0[m] : 1.0) * gam[v];
+        if (n_terms > 1) o -= (t1 ? t1[m] : 1.0) * gam[k + v];
+        Y[m + static_cast<i64>(v) * ldy] = o;
+    }
+}
+
+}  // namespace
+
+int spmm_padded_k(int k, bool f32, int* lanes_per_entry) {
+    const int vn = f32 ? 4 : 2;
+    int lpe = 1;
+    while (lpe * vn < k) lpe *= 2;
+    if (lpe > 64) throw std::runtime_error("multi-vector product wider than the wavefront");
+    if (lanes_per_entry) *lanes_per_entry = lpe;
+    return lpe * vn;
+}
+
+void launch_spmm_stage(const double* X, i64 ld, i64 L, int k, int kp, const double* xscale, bool f32, void* out,
+                       hipStream_t s) {
+    if (L <= 0) return;
+    const i64 grid = std::max<i64>(1, std::min<i64>((L * kp + 255) / 256, 16384));
+    if (f32)
+        hipLaunchKernelGGL(k_spmm_stage<float>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, X, ld, L, k, kp,
+                           xscale, static_cast<float*>(out));
+    else
+        hipLaunchKernelGGL(k_spmm_stage<double>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, X, ld, L, k, kp,
+                           xscale, static_cast<double*>(out));
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+void launch_spmm_lines(const TiledLines& t, const void* X, int k, bool x_f32, double* Y, hipStream_t s) {
+    if (t.n_major <= 0) return;
+    int lpe = 1;
+    const int kp = spmm_padded_k(k, x_f32, &lpe);
+    if (t.st == ST_F64 && !x_f32) spmm_dispatch<double, double>(t, X, kp, lpe, Y, s);
+    else if (t.st == ST_F32 && x_f32) spmm_dispatch<float, float>(t, X, kp, lpe, Y, s);
+    else if (t.st == ST_F32) spmm_dispatch<float, double>(t, X, kp, lpe, Y, s);
+    else throw std::runtime_error("multi-vector products need FP64 or FP32 sparse storage");
+}
+
+void launch_spmm_finish(const double* Yrm, i64 n, int k, int kp, const double* oscale, int n_terms,
+                        const double* t0, const double* t1, const double* gam, double* Y, i64 ldy, hipStream_t s) {
+    if (n <= 0 || k <= 0) return;
+    const i64 grid = std::max<i64>(1, std::min<i64>((n * k + 255) / 256, 16384));
+    hipLaunchKernelGGL(k_spmm_finish, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, Yrm, n, k, kp, oscale,
+                       n_terms, t0, t1, gam, Y, ldy);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+}  // namespace irlba_hip

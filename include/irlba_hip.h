@@ -116,6 +116,15 @@ int irlba_hip_matrix_dims(const irlba_hip_matrix* m, int64_t* rows, int64_t* col
 int irlba_hip_multiply(const irlba_hip_matrix* m, int adjoint, const double* right, double* output,
                        int device_pointers);
 
+/* Block product: output (k columns) = M right (adjoint 0) or M' right (adjoint 1)
+ * for k column-major vectors at once, 1 <= k <= 128.  No reference counterpart
+ * (the reference multiplies one vector per call, interface.hpp:49,87); sparse
+ * handles stream the matrix once for all k, which block methods and the SVD
+ * backward's CG use.  Same precision as irlba_hip_multiply except that sparse
+ * products accumulate in FP64. */
+int irlba_hip_multiply_block(const irlba_hip_matrix* m, int adjoint, int64_t k, const double* right, double* output,
+                             int device_pointers);
+
 /* RealizeWorkspace::realize_copy (include/irlba/Matrix/interface.hpp:119-130):
  * rows x cols, column-major FP64. */
 int irlba_hip_realize(const irlba_hip_matrix* m, double* buffer, int device_pointers);

@@ -65,3 +65,50 @@ def test_lanczos_sweep_vs_oracle(nr, nc, work, start):
     np.testing.assert_allclose(hF, oF, rtol=1e-8, atol=1e-10)
     np.testing.assert_allclose(hW.T @ hW, np.eye(work), atol=1e-11)
     np.testing.assert_allclose(hV.T @ hV, np.eye(work), atol=1e-11)
+
+
+def _csc_dense(vals, idx, ptr, nr, nc):
+    A = np.zeros((nr, nc))
+    for c in range(nc):
+        A[idx[ptr[c]:ptr[c + 1]], c] = vals[ptr[c]:ptr[c + 1]]
+    return A
+
+
+@pytest.mark.parametrize("storage,accumulate", [("f64", "f64"), ("f32", "f32"), ("f32", "f64")])
+@pytest.mark.parametrize("k", [1, 3, 16, 33])
+@pytest.mark.parametrize("decorated", [False, True])
+def test_multiply_block_vs_numpy(storage, accumulate, k, decorated):
+    # Block products on the tiled layout: 70000 columns span three FP32 tiles of
+    # the row orientation; centring + scaling folded in as for the single product.
+    nr, nc = 300, 70000
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.02, 3, f32=False)
+    A = _csc_dense(vals, idx, ptr, nr, nc)
+    H = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, storage=storage, accumulate=accumulate)
+    ref_A = A.astype(np.float32).astype(np.float64) if storage == "f32" else A
+    M = H
+    if decorated:
+        c, s = irlba.column_stats(H)
+        M = irlba.ScaledMatrix(irlba.CenteredMatrix(H, c), s)
+        ref_A = (ref_A - c[None, :]) / s[None, :]
+    rng = np.random.default_rng(k)
+    X = rng.standard_normal((nc, k))
+    U = rng.standard_normal((nr, k))
+    tol = 1e-12 if storage == "f64" else (2e-6 if accumulate == "f32" else 1e-12)
+    Y = M.multiply_block(X)
+    Z = M.multiply_block(U, adjoint=True)
+    assert np.linalg.norm(Y - ref_A @ X) <= tol * np.linalg.norm(ref_A @ X)
+    assert np.linalg.norm(Z - ref_A.T @ U) <= tol * np.linalg.norm(ref_A.T @ U)
+    # Each column equals the single-vector product to the accumulation's rounding.
+    y0 = M.multiply(X[:, 0])
+    assert np.linalg.norm(Y[:, 0] - y0) <= max(tol, 1e-12) * np.linalg.norm(y0)
+
+
+def test_multiply_block_dense_and_errors():
+    A = np.random.default_rng(1).standard_normal((50, 40))
+    H = irlba.HipDenseMatrix(A)
+    X = np.random.default_rng(2).standard_normal((40, 5))
+    np.testing.assert_allclose(H.multiply_block(X), A @ X, rtol=1e-12, atol=1e-12)
+    with pytest.raises(ValueError):
+        H.multiply_block(np.ones((39, 2)))
+    with pytest.raises(irlba.HipError, match="between 1 and 128"):
+        H.multiply_block(np.ones((40, 129)))
