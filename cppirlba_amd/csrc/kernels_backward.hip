@@ -12,8 +12,8 @@
 //
 // The small dense products of the backward (U'G, the centre sums) are
 // tall-skinny Gram products A'B with both operands L x k, k <= 64: blocks over
-// row ranges stage 32-row tiles of both in LDS and write k x k partials that a
-// second kernel sums in a fixed order.
+// ~64-row ranges stage 32-row tiles of both in LDS and write k x k partials
+// that a second kernel sums in a fixed order.
 #include "common.hpp"
 
 namespace irlba_hip {
@@ -138,7 +138,8 @@ constexpr int kGramRows = 32;
 constexpr int kGramMaxK = 64;
 constexpr int kGramPad = kGramRows + 1;  // LDS row stride (breaks bank aliasing)
 
-int gram_blocks(i64 L) { return static_cast<int>(std::max<i64>(1, std::min<i64>(256, (L + 255) / 256))); }
+// About 64 rows (two LDS tiles) per block, at most 1024 blocks.
+int gram_blocks(i64 L) { return static_cast<int>(std::max<i64>(1, std::min<i64>(1024, (L + 63) / 64))); }
 
 __global__ __launch_bounds__(256) void k_gram(const double* __restrict__ A, i64 lda, const double* __restrict__ B,
                                               i64 ldb, int k, i64 L, i64 chunk, double* __restrict__ partials) {
@@ -181,13 +182,18 @@ __global__ __launch_bounds__(256) void k_gram(const double* __restrict__ A, i64 
 }
 
 // out[i * ld_out + j] = sum over blocks of partials[b][i * k + j] = (A'B)(i, j):
-// row-major, the layout of the rotation coefficients.
-__global__ void k_gram_reduce(const double* __restrict__ partials, int nb, int k, double* __restrict__ out,
-                              int ld_out) {
+// row-major, the layout of the rotation coefficients.  One wavefront per output,
+// lanes over the blocks in a fixed order.
+__global__ __launch_bounds__(256) void k_gram_reduce(const double* __restrict__ partials, int nb, int k,
+                                                     double* __restrict__ out, int ld_out) {
     const int kk = k * k;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < kk; p += gridDim.x * blockDim.x) {
-        double s = 0;
-        for (int b = 0; b < nb; ++b) s += partials[static_cast<i64>(b) * kk + p];
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= kk) return;  // whole wavefronts only
+    double s = 0;
+    for (int b = lane; b < nb; b += kWave) s += partials[static_cast<i64>(b) * kk + p];
+    s = wave_sum(s);
+    if (lane == 0) {
         const int i = p / k, j = p - i * k;
         out[static_cast<i64>(i) * ld_out + j] = s;
     }
@@ -201,7 +207,7 @@ void launch_gram(const double* A, i64 lda, const double* B, i64 ldb, int k, i64 
     const i64 chunk = (L + nb - 1) / nb;
     hipLaunchKernelGGL(k_gram, dim3(nb), dim3(256), 0, s, A, lda, B, ldb, k, L, chunk, partials);
     IRLBA_HIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gram_reduce, dim3((k * k + 255) / 256), dim3(256), 0, s, partials, nb, k, out, ld_out);
+    hipLaunchKernelGGL(k_gram_reduce, dim3((k * k + 3) / 4), dim3(256), 0, s, partials, nb, k, out, ld_out);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 

@@ -52,7 +52,27 @@ __global__ __launch_bounds__(256) void k_spmm_lines(const i64* __restrict__ ptr,
             const i64 seg = static_cast<i64>(t) * n_major + m;
             const i64 s0 = ptr[seg], s1 = ptr[seg + 1];
             const i64 base = static_cast<i64>(t) * tile_len;
-            for (i64 s = s0 + g; s < s1; s += G) {
+            // Four entries per group in flight: their value/index loads, then
+            // their gathers, are issued back to back.
+            i64 s = s0 + g;
+            for (; s + 3 * G < s1; s += 4 * G) {
+                double a[4];
+                i64 r[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    a[u] = to_double_any(__builtin_nontemporal_load(val + s + u * G));
+                    r[u] = base + idx[s + u * G];
+                }
+                PT x[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)  // padding slots (value 0) gather row `base`, a valid row
+                    x[u] = *reinterpret_cast<const PT*>(X + r[u] * kp + q * VN);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < VN; ++v) acc[v] += a[u] * static_cast<double>(x[u][v]);
+            }
+            for (; s < s1; s += G) {
                 const double a = to_double_any(__builtin_nontemporal_load(val + s));
                 if (a == 0) continue;  // padding slots (and stored zeros) contribute nothing
                 const i64 r = base + idx[s];
