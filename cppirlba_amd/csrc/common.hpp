@@ -229,6 +229,11 @@ void launch_spmm_stage(const double* X, i64 ld, i64 L, int k, int kp, const doub
 void launch_spmm_lines(const TiledLines& t, const void* X, int k, bool x_f32, double* Y, hipStream_t s);
 void launch_spmm_finish(const double* Yrm, i64 n, int k, int kp, const double* oscale, int n_terms,
                         const double* t0, const double* t1, const double* gam, double* Y, i64 ldy, hipStream_t s);
+// Dense block products on packed inputs (kp <= 32); `scratch` (FP64) holds the
+// per-chunk partials of the axpy orientation, up to dense_block_scratch(d, kp).
+i64 dense_block_scratch(const DenseDesc& d, int kp);
+void launch_dense_block(const DenseDesc& d, bool adjoint, int kp, bool x_f32, const void* X, double* Y,
+                        double* scratch, i64 scratch_cap, hipStream_t s);
 
 // Synthetic data (counter-based, identical to the oracle's generator).
 void launch_gen_nb_count(i64 n_minor, i64 n_major, std::uint32_t thr, std::uint64_t seed, i64* counts,

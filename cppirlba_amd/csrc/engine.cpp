@@ -293,22 +293,25 @@ void apply_handle(const irlba_hip_matrix& m, bool adjoint, const double* x, doub
 }
 
 // Y[:, :k] = M X[:, :k] (or M' X) for k column vectors (column-major, leading
-// dimensions ldx, ldy).  Sparse handles stream the matrix once for all k
-// (kernels_spmm.hip); dense ones run k single products.
+// dimensions ldx, ldy).  Sparse and dense handles stream the matrix once for all
+// k (kernels_spmm.hip; dense up to 32 vectors per pass); otherwise k single
+// products.
 void apply_handle_multi(const irlba_hip_matrix& m, bool adjoint, const double* X, i64 ldx, double* Y, i64 ldy,
                         int k) {
     auto* sb = dynamic_cast<SparseBase*>(m.base.get());
-    if (!sb || k == 1) {
+    auto* db = dynamic_cast<DenseBase*>(m.base.get());
+    const bool dense_x_f32 = db && db->d.st != ST_F64;
+    if (k == 1 || (!sb && !db) || (db && spmm_padded_k(k, dense_x_f32) > 32)) {
         for (int v = 0; v < k; ++v) apply_handle(m, adjoint, X + v * ldx, Y + v * ldy, nullptr, nullptr);
         return;
     }
     irlba_hip_context* ctx = m.ctx;
     hipStream_t s = ctx->stream;
     const Affine& a = m.aff;
-    const TiledLines& t = adjoint ? sb->by_col : sb->by_row;
-    const bool x_f32 = t.st == ST_F32 && !sb->acc64;
+    const bool x_f32 = sb ? ((adjoint ? sb->by_col : sb->by_row).st == ST_F32 && !sb->acc64) : dense_x_f32;
     const int kp = spmm_padded_k(k, x_f32);
-    const i64 L_in = adjoint ? sb->nr : sb->nc, L_out = adjoint ? sb->nc : sb->nr;
+    const i64 L_in = adjoint ? m.base->nr : m.base->nc, L_out = adjoint ? m.base->nc : m.base->nr;
+    const i64 dscratch = db ? dense_block_scratch(db->d, kp) : 0;
     const double* xscale = adjoint ? a.dr.get() : a.dc.get();
     const double* oscale = adjoint ? a.dc.get() : a.dr.get();
     const double* tdot[kMaxTerms] = {nullptr, nullptr};
@@ -326,6 +329,7 @@ void apply_handle_multi(const irlba_hip_matrix& m, bool adjoint, const double* X
         p.push_back(c.take<double>(2 * k));                                            // gammas
         p.push_back(c.take<double>(static_cast<i64>(cgs_dot_blocks(L_in)) * k));      // dot partials
         p.push_back(need_ones ? c.take<double>(L_in) : nullptr);                       // ones
+        p.push_back(db ? c.take<double>(dscratch) : nullptr);                          // dense partials
     };
     std::vector<void*> wp;
     Carve sizing;
@@ -343,7 +347,11 @@ void apply_handle_multi(const irlba_hip_matrix& m, bool adjoint, const double* X
         launch_cgs_reduce(partials, L_in, k, gam + j * k, s);
     }
     launch_spmm_stage(X, ldx, L_in, k, kp, xscale, x_f32, wp[0], s);
-    launch_spmm_lines(t, wp[0], k, x_f32, static_cast<double*>(wp[1]), s);
+    if (sb)
+        launch_spmm_lines(adjoint ? sb->by_col : sb->by_row, wp[0], k, x_f32, static_cast<double*>(wp[1]), s);
+    else
+        launch_dense_block(db->d, adjoint, kp, x_f32, wp[0], static_cast<double*>(wp[1]), static_cast<double*>(wp[5]),
+                           dscratch, s);
     launch_spmm_finish(static_cast<const double*>(wp[1]), L_out, k, kp, oscale, a.n_terms, tvec[0], tvec[1], gam, Y,
                        ldy, s);
     if (adjoint) ctx->stats.adjoint_products += k; else ctx->stats.forward_products += k;

@@ -146,7 +146,149 @@ __global__ void k_spmm_finish(const double* __restrict__ Yrm, i64 n, int k, int 
     }
 }
 
+// ------------------------------------------------------------------ dense blocks
+// Dense storage loads one element at a time here (the block products serve the
+// small dense cases of the backward's CG, where launches, not bytes, dominate).
+template <typename VT>
+__device__ __forceinline__ double dense_at(const VT* A, i64 i) {
+    return to_double_any(A[i]);
+}
+
+// Y[m * kp + v] = sum_j A[m, j] X[j * kp + v] over lines m of length Ln (A's
+// lines contiguous): one block per pair of lines, threads over j, a block-wide
+// reduction of the 2 x kp sums.
+template <typename VT, typename FT, int KP>
+__global__ __launch_bounds__(256) void k_dense_block_dot(const VT* __restrict__ A, i64 n_lines, i64 Ln,
+                                                         const FT* __restrict__ X, double* __restrict__ Y) {
+    __shared__ double red[4][2 * KP];
+    const i64 m0 = static_cast<i64>(blockIdx.x) * 2;
+    const bool two = m0 + 1 < n_lines;
+    double acc[2][KP];
+#pragma unroll
+    for (int v = 0; v < KP; ++v) acc[0][v] = acc[1][v] = 0;
+    for (i64 j = threadIdx.x; j < Ln; j += 256) {
+        const double a0 = dense_at(A, m0 * Ln + j);
+        const double a1 = two ? dense_at(A, (m0 + 1) * Ln + j) : 0.0;
+        const FT* xr = X + j * KP;
+#pragma unroll
+        for (int v = 0; v < KP; ++v) {
+            const double x = static_cast<double>(xr[v]);
+            acc[0][v] += a0 * x;
+            acc[1][v] += a1 * x;
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int v = 0; v < KP; ++v) {
+            const double t = wave_sum(acc[r][v]);
+            if (lane == 0) red[w][r * KP + v] = t;
+        }
+    __syncthreads();
+    if (threadIdx.x < 2 * KP) {
+        const int r = threadIdx.x / KP, v = threadIdx.x - r * KP;
+        if (r == 0 || two)
+            Y[(m0 + r) * KP + v] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+    }
+}
+
+// P[c][i * kp + v] = sum over lines l of chunk c of A[l, i] U[l * kp + v]
+// (thread per i); the chunks are summed by k_dense_block_fold.
+template <typename VT, typename FT, int KP>
+__global__ __launch_bounds__(256) void k_dense_block_axpy(const VT* __restrict__ A, i64 n_lines, i64 Ln, i64 chunk,
+                                                          const FT* __restrict__ U, double* __restrict__ P) {
+    const i64 i = static_cast<i64>(blockIdx.x) * 256 + threadIdx.x;
+    const i64 l0 = static_cast<i64>(blockIdx.y) * chunk, l1 = min(n_lines, l0 + chunk);
+    if (i >= Ln) return;
+    double acc[KP];
+#pragma unroll
+    for (int v = 0; v < KP; ++v) acc[v] = 0;
+    for (i64 l = l0; l < l1; ++l) {
+        const double a = dense_at(A, l * Ln + i);
+        const FT* ur = U + l * KP;  // the same row for every thread: a broadcast load
+#pragma unroll
+        for (int v = 0; v < KP; ++v) acc[v] += a * static_cast<double>(ur[v]);
+    }
+    double* out = P + static_cast<i64>(blockIdx.y) * Ln * KP + i * KP;
+#pragma unroll
+    for (int v = 0; v < KP; ++v) out[v] = acc[v];
+}
+
+__global__ void k_dense_block_fold(const double* __restrict__ P, int chunks, i64 n, double* __restrict__ Y) {
+    for (i64 e = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<i64>(gridDim.x) * blockDim.x) {
+        double s = 0;
+        for (int c = 0; c < chunks; ++c) s += P[static_cast<i64>(c) * n + e];
+        Y[e] = s;
+    }
+}
+
+template <typename VT, typename FT, int KP>
+void dense_block_typed(const DenseDesc& d, bool adjoint, const void* Xv, double* Y, double* scratch, i64 scratch_cap,
+                       hipStream_t s) {
+    const VT* A = static_cast<const VT*>(d.a);
+    const FT* X = static_cast<const FT*>(Xv);
+    const i64 n_lines = d.col_major ? d.nc : d.nr;
+    const i64 Ln = d.col_major ? d.nr : d.nc;
+    const bool dot = d.col_major ? adjoint : !adjoint;
+    if (dot) {
+        const i64 grid = (n_lines + 1) / 2;
+        hipLaunchKernelGGL((k_dense_block_dot<VT, FT, KP>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, A,
+                           n_lines, Ln, X, Y);
+        IRLBA_HIP_LAUNCH_CHECK();
+        return;
+    }
+    const i64 xb = (Ln + 255) / 256;
+    i64 chunks = std::max<i64>(1, std::min<i64>(std::max<i64>(1, 1024 / xb), (n_lines + 31) / 32));
+    chunks = std::min<i64>(chunks, std::max<i64>(1, scratch_cap / std::max<i64>(1, Ln * KP)));
+    const i64 chunk = (n_lines + chunks - 1) / chunks;
+    chunks = (n_lines + chunk - 1) / chunk;
+    hipLaunchKernelGGL((k_dense_block_axpy<VT, FT, KP>), dim3(static_cast<unsigned>(xb), static_cast<unsigned>(chunks)),
+                       dim3(256), 0, s, A, n_lines, Ln, chunk, X, scratch);
+    IRLBA_HIP_LAUNCH_CHECK();
+    const i64 n = Ln * KP;
+    hipLaunchKernelGGL(k_dense_block_fold, dim3(static_cast<unsigned>(std::min<i64>((n + 255) / 256, 4096))), dim3(256),
+                       0, s, scratch, static_cast<int>(chunks), n, Y);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+template <typename VT, typename FT>
+void dense_block_kp(const DenseDesc& d, bool adjoint, int kp, const void* X, double* Y, double* scratch, i64 cap,
+                    hipStream_t s) {
+    switch (kp) {
+        case 2: dense_block_typed<VT, FT, 2>(d, adjoint, X, Y, scratch, cap, s); break;
+        case 4: dense_block_typed<VT, FT, 4>(d, adjoint, X, Y, scratch, cap, s); break;
+        case 8: dense_block_typed<VT, FT, 8>(d, adjoint, X, Y, scratch, cap, s); break;
+        case 16: dense_block_typed<VT, FT, 16>(d, adjoint, X, Y, scratch, cap, s); break;
+        case 32: dense_block_typed<VT, FT, 32>(d, adjoint, X, Y, scratch, cap, s); break;
+        default: throw std::runtime_error("dense block products take at most 32 columns per pass");
+    }
+}
+
 }  // namespace
+
+i64 dense_block_scratch(const DenseDesc& d, int kp) {
+    const i64 n_lines = d.col_major ? d.nc : d.nr;
+    const i64 Ln = d.col_major ? d.nr : d.nc;
+    const i64 xb = (Ln + 255) / 256;
+    const i64 chunks = std::max<i64>(1, std::min<i64>(std::max<i64>(1, 1024 / xb), (n_lines + 31) / 32));
+    return Ln * kp * chunks;  // the axpy orientation's partials (the dot one needs none)
+}
+
+void launch_dense_block(const DenseDesc& d, bool adjoint, int kp, bool x_f32, const void* X, double* Y,
+                        double* scratch, i64 scratch_cap, hipStream_t s) {
+    if (d.st == ST_F64) {
+        if (x_f32) throw std::runtime_error("FP64 dense storage takes FP64 block inputs");
+        dense_block_kp<double, double>(d, adjoint, kp, X, Y, scratch, scratch_cap, s);
+    } else if (d.st == ST_F32) {
+        if (!x_f32) throw std::runtime_error("FP32 dense storage takes FP32 block inputs");
+        dense_block_kp<float, float>(d, adjoint, kp, X, Y, scratch, scratch_cap, s);
+    } else {
+        if (!x_f32) throw std::runtime_error("BF16 dense storage takes FP32 block inputs");
+        dense_block_kp<__hip_bfloat16, float>(d, adjoint, kp, X, Y, scratch, scratch_cap, s);
+    }
+}
 
 int spmm_padded_k(int k, bool f32, int* lanes_per_entry) {
     const int vn = f32 ? 4 : 2;

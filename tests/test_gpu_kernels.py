@@ -103,11 +103,39 @@ def test_multiply_block_vs_numpy(storage, accumulate, k, decorated):
     assert np.linalg.norm(Y[:, 0] - y0) <= max(tol, 1e-12) * np.linalg.norm(y0)
 
 
+@pytest.mark.parametrize("storage", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("order", ["C", "F"])
+@pytest.mark.parametrize("k", [3, 16, 33])
+def test_multiply_block_dense(storage, order, k):
+    # Both orientations of both layouts (dot-lines and axpy-lines with chunk
+    # partials); k = 33 exceeds one pass and falls back to single products.
+    rng = np.random.default_rng(k)
+    A = np.asarray(rng.standard_normal((700, 300)), order=order)
+    H = irlba.HipDenseMatrix(A.astype(np.float32) if storage == "f32" else A, storage=storage)
+    if storage == "bf16":
+        import torch
+        ref = torch.tensor(A).to(torch.bfloat16).double().numpy()
+        tol = 1e-6  # the inputs are rounded to FP32, the stored values to BF16 (in ref)
+    elif storage == "f32":
+        ref = A.astype(np.float32).astype(np.float64)
+        tol = 1e-6
+    else:
+        ref = A
+        tol = 1e-12
+    X = rng.standard_normal((300, k))
+    U = rng.standard_normal((700, k))
+    assert np.linalg.norm(H.multiply_block(X) - ref @ X) <= tol * np.linalg.norm(ref @ X)
+    assert np.linalg.norm(H.multiply_block(U, adjoint=True) - ref.T @ U) <= tol * np.linalg.norm(ref.T @ U)
+
+
 def test_multiply_block_dense_and_errors():
     A = np.random.default_rng(1).standard_normal((50, 40))
     H = irlba.HipDenseMatrix(A)
     X = np.random.default_rng(2).standard_normal((40, 5))
     np.testing.assert_allclose(H.multiply_block(X), A @ X, rtol=1e-12, atol=1e-12)
+    c, s = irlba.column_stats(H)
+    D = irlba.ScaledMatrix(irlba.CenteredMatrix(H, c), s)
+    np.testing.assert_allclose(D.multiply_block(X), ((A - c) / s) @ X, rtol=1e-12, atol=1e-12)
     with pytest.raises(ValueError):
         H.multiply_block(np.ones((39, 2)))
     with pytest.raises(irlba.HipError, match="between 1 and 128"):
