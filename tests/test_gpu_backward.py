@@ -191,3 +191,15 @@ def test_backward_argument_errors():
     w = np.array([1.0, -2.0, 0.5])
     res = irlba.svd_backward(h, U, s, V, grad_D=w, complement="none")
     np.testing.assert_allclose(res.dense(), U @ np.diag(w) @ V.T, atol=1e-13)
+
+
+@pytest.mark.parametrize("name", ["tall", "wide"])
+def test_engine_matches_torch_fixture(name):
+    # The engine's exact backward against the committed torch.linalg.svd derivative.
+    import os
+
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", "svd_backward_torch.npz"))
+    g = {k[len(name) + 1:]: f[k] for k in f.files if k.startswith(name + "_")}
+    res = irlba.svd_backward(irlba.HipDenseMatrix(g["A"]), g["U"], g["s"], g["V"], g["w"], g["GU"], g["GV"],
+                             complement="exact")
+    assert _close(res.dense(), g["dA"], 1e-9)

@@ -101,3 +101,14 @@ def test_center_gradient_matches_finite_differences():
         cm[j] -= eps
         fd[j] = (B.loss(Bm - cp[None, :], k, w, GU, GV, U, V) - B.loss(Bm - cm[None, :], k, w, GU, GV, U, V)) / (2 * eps)
     assert np.abs(g - fd).max() < 1e-6 * max(1.0, np.abs(fd).max())
+
+
+@pytest.mark.parametrize("name", ["tall", "wide"])
+def test_oracle_matches_torch_fixture(name):
+    # Committed vectors from torch.linalg.svd's autograd (tests/golden/make_backward_fixtures.py).
+    import os
+
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", "svd_backward_torch.npz"))
+    g = {k[len(name) + 1:]: f[k] for k in f.files if k.startswith(name + "_")}
+    X, Y = B.svd_backward(g["A"], g["U"], g["s"], g["V"], g["w"], g["GU"], g["GV"], "exact")
+    np.testing.assert_allclose(X @ Y.T, g["dA"], atol=1e-10 * max(1.0, np.abs(g["dA"]).max()))
