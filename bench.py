@@ -129,7 +129,8 @@ def spmv_mode(args):
                       "forward_ms": st["forward_kernel_ms"] / max(1, st["forward_products"]),
                       "adjoint_ms": st["adjoint_kernel_ms"] / max(1, st["adjoint_products"]),
                       "forward_bytes": st["forward_bytes"], "adjoint_bytes": st["adjoint_bytes"],
-                      "copy_bytes": n * 8}), flush=True)
+                      "copy_bytes": n * 8, "forward_layout": mat.layout(False),
+                      "adjoint_layout": mat.layout(True)}), flush=True)
 
 
 def _sparse_uniform_csc(torch, m, n, density, gen, dev, chunk=1000):

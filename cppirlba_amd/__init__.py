@@ -207,6 +207,15 @@ class _Handle:
         check(lib().irlba_hip_multiply_block(self.handle, int(adjoint), k, X.ctypes.data, out.ctypes.data, 0))
         return out
 
+    def layout(self, adjoint=False):
+        """Sparse handles: the device layout that A.v (adjoint=False) or A'.u streams, as a dict
+        (irlba_hip_sparse_layout)."""
+        from ._lib import Layout
+
+        out = Layout()
+        check(lib().irlba_hip_sparse_layout(self.handle, int(adjoint), ctypes.byref(out)))
+        return {name: getattr(out, name) for name, _ in Layout._fields_}
+
     def realize(self):
         """RealizeWorkspace::realize_copy (interface.hpp:125-130)."""
         nr, nc = self.shape
@@ -241,6 +250,7 @@ class HipSparseMatrix(_Handle):
         check(lib().irlba_hip_sparse_create(ctx.handle, nrow, ncol, in_t, xp, ip, pp, int(column_major), st,
                                             int(acc == F64), int(xdev), ctypes.byref(h)))
         super().__init__(ctx, h)
+
 
 
 class HipDenseMatrix(_Handle):

@@ -47,6 +47,20 @@ class Stats(ctypes.Structure):
     ]
 
 
+class Layout(ctypes.Structure):
+    _fields_ = [
+        ("nnz", ctypes.c_int64),
+        ("slots", ctypes.c_int64),
+        ("index_bytes", ctypes.c_int64),
+        ("tile_len", ctypes.c_int32),
+        ("n_tiles", ctypes.c_int32),
+        ("vec", ctypes.c_int32),
+        ("lanes_per_line", ctypes.c_int32),
+        ("delta", ctypes.c_int32),
+        ("blocks", ctypes.c_int32),
+    ]
+
+
 MATVEC_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                              ctypes.POINTER(ctypes.c_double))
 
@@ -126,4 +140,5 @@ SIGNATURES.update({
                                     _VP, _VP, _VP, _I32, _i]),
     "irlba_hip_sddmm": (_i, [_VP, _i64, _i64, _VP, _VP, _i, _i64, _VP, _VP, _VP, _i, _i]),
     "irlba_hip_multiply_block": (_i, [_VP, _i, _i64, _VP, _VP, _i]),
+    "irlba_hip_sparse_layout": (_i, [_VP, _i, ctypes.POINTER(Layout)]),
 })

@@ -49,6 +49,59 @@ __device__ __forceinline__ T group_sum(T v) {
     return v;
 }
 
+// Inclusive prefix sum within aligned groups of `Width` lanes.
+template <int Width, typename T>
+__device__ __forceinline__ T group_scan_incl(T v) {
+    const int l = threadIdx.x & (Width - 1);
+#pragma unroll
+    for (int off = 1; off < Width; off <<= 1) {
+        const T u = __shfl_up(v, off, Width);
+        if (l >= off) v += u;
+    }
+    return v;
+}
+
+// The same scan for 32-bit integers with DPP lane moves (no LDS traffic): row
+// shifts by 1, 2, 4, 8 inside 16-lane rows (a select keeps groups narrower
+// than a row apart), then the row broadcasts of lane 15 and lane 31.
+template <int Width>
+__device__ __forceinline__ unsigned group_scan_incl_dpp(unsigned v) {
+    static_assert(Width >= 2 && Width <= 64 && (Width & (Width - 1)) == 0, "group width");
+    const int l = threadIdx.x & 15;
+    int x = static_cast<int>(v);
+    int u;
+    u = __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);  // row_shr:1
+    x += (Width >= 16 || (l & (Width - 1)) >= 1) ? u : 0;
+    if (Width >= 4) {
+        u = __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);  // row_shr:2
+        x += (Width >= 16 || (l & (Width - 1)) >= 2) ? u : 0;
+    }
+    if (Width >= 8) {
+        u = __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);  // row_shr:4
+        x += (Width >= 16 || (l & (Width - 1)) >= 4) ? u : 0;
+    }
+    if (Width >= 16) {
+        u = __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);  // row_shr:8
+        x += u;
+    }
+    if (Width >= 32) {
+        u = __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+        x += u;
+    }
+    if (Width >= 64) {
+        u = __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+        x += u;
+    }
+    return static_cast<unsigned>(x);
+}
+
+// Value of the last lane of each aligned group of `Width` lanes.
+template <int Width>
+__device__ __forceinline__ unsigned group_last(unsigned v) {
+    if constexpr (Width == 64) return static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+    else return static_cast<unsigned>(__shfl(static_cast<int>(v), Width - 1, Width));
+}
+
 // Block-wide sum of up to two doubles; every thread gets the totals.
 // `scratch` needs 2 * (blockDim.x / 64) doubles of LDS.
 __device__ __forceinline__ void block_sum2(double& a, double& b, double* scratch) {
@@ -144,6 +197,11 @@ struct TiledLines {
     std::uint16_t* idx = nullptr;
     void* val = nullptr;
     i64* chunk = nullptr;       // device, B+1 segment boundaries (nnz quantiles)
+    // Delta layout (vec == 4): one byte per slot instead of `idx` (kernels_sparse.hip,
+    // "8-bit delta indices"); `idx` is then only filled on demand by launch_delta_expand.
+    bool delta = false;
+    std::uint8_t* dlt = nullptr;
+    std::uint16_t* first = nullptr;  // device, n_tiles*n_major
 };
 
 // ------------------------------------------------------------------ launchers (kernels_*.hip)
@@ -160,6 +218,14 @@ void launch_sort_segments(const i64* ptr, const unsigned long long* counts, i64 
                           const std::uint16_t* in_idx, const void* in_val, std::uint16_t* out_idx, void* out_val,
                           StorageType st, int* dup_flag, hipStream_t s);
 void launch_chunk_bounds(const i64* ptr, i64 nseg, int B, i64* chunk, hipStream_t s);
+// Delta layout: slot counts with the bridging slots, the conversion from the
+// sorted 16-bit layout, and the expansion back to 16-bit indices.
+void launch_delta_count(const i64* ptrA, const unsigned long long* counts, i64 nseg, const std::uint16_t* idxA,
+                        unsigned long long* lenB, hipStream_t s);
+void launch_delta_write(const i64* ptrA, const unsigned long long* counts, i64 nseg, const std::uint16_t* idxA,
+                        const void* valA, StorageType st, const i64* ptrB, std::uint16_t* first, std::uint8_t* dltB,
+                        void* valB, hipStream_t s);
+void launch_delta_expand(const TiledLines& t, std::uint16_t* idx, hipStream_t s);
 // Sparse products.
 void launch_spmv_tiled(const TiledLines& t, const Prologue& pro, const Epilogue& epi, bool acc64, void* partial,
                        double* gamma_part, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);

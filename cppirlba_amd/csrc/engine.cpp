@@ -198,7 +198,19 @@ struct SparseBase : Base {
     }
     void realize(double* dense) override {
         IRLBA_HIP_CHECK(hipMemsetAsync(dense, 0, sizeof(double) * nr * nc, ctx->stream));
-        launch_sparse_realize(by_col, true, nr, dense, ctx->stream);
+        launch_sparse_realize(indexed(true), true, nr, dense, ctx->stream);
+    }
+    // The orientation with 16-bit indices, expanded once from the delta layout
+    // for the kernels that read indices out of order (realize, block products).
+    const TiledLines& indexed(bool adjoint) {
+        TiledLines& t = adjoint ? by_col : by_row;
+        if (t.delta && !t.idx && t.slots > 0) {
+            auto idx = dalloc<std::uint16_t>(t.slots);
+            launch_delta_expand(t, idx.get(), ctx->stream);
+            t.idx = idx.get();
+            keep.push_back(idx);
+        }
+        return t;
     }
     void colstats(double* c, double* s) override { launch_sparse_colstats(by_col, nr, c, s, ctx->stream); }
     double product_bytes(bool adjoint) const override {
@@ -348,7 +360,7 @@ void apply_handle_multi(const irlba_hip_matrix& m, bool adjoint, const double* X
     }
     launch_spmm_stage(X, ldx, L_in, k, kp, xscale, x_f32, wp[0], s);
     if (sb)
-        launch_spmm_lines(adjoint ? sb->by_col : sb->by_row, wp[0], k, x_f32, static_cast<double*>(wp[1]), s);
+        launch_spmm_lines(sb->indexed(adjoint), wp[0], k, x_f32, static_cast<double*>(wp[1]), s);
     else
         launch_dense_block(db->d, adjoint, kp, x_f32, wp[0], static_cast<double*>(wp[1]), static_cast<double*>(wp[5]),
                            dscratch, s);
@@ -426,6 +438,45 @@ TiledLines build_lines(irlba_hip_context* ctx, const std::int32_t* d_idx, const 
     require(dup_h == 0, "sparse input holds the same row/column pair more than once");
     sidx.reset();
     sval.reset();
+
+    // 8-bit delta indices: one byte per slot instead of two, plus a bridging
+    // slot (value 0) for each in-tile gap above 255.  Kept when the bridging
+    // slots cost less than the byte they save (3% margin), i.e. for segments
+    // of mean gap well under 255.
+    std::shared_ptr<std::uint8_t> dlt;
+    std::shared_ptr<std::uint16_t> first;
+    // IRLBA_HIP_DELTA=0 keeps the 16-bit layout, =2 takes the delta layout whatever it costs.
+    const char* de = std::getenv("IRLBA_HIP_DELTA");
+    const int delta_mode = de ? std::atoi(de) : 1;
+    if (t.vec == 4 && delta_mode != 0) {
+        auto lenB = dalloc<unsigned long long>(nseg);
+        launch_delta_count(ptr.get(), counts.get(), nseg, fidx.get(), lenB.get(), s);
+        auto plenB = dalloc<i64>(nseg);
+        launch_pad_counts(lenB.get(), plenB.get(), nseg, seg_align, s);
+        lenB.reset();
+        auto ptrB = dalloc<i64>(nseg + 1);
+        exclusive_scan_i64(plenB.get(), ptrB.get(), nseg, s);
+        plenB.reset();
+        i64 slotsB = 0;
+        IRLBA_HIP_CHECK(hipMemcpy(&slotsB, ptrB.get() + nseg, sizeof(i64), hipMemcpyDeviceToHost));
+        const double bytes_u16 = static_cast<double>(t.slots) * (vb + 2.0);
+        const double bytes_u8 = static_cast<double>(slotsB) * (vb + 1.0);
+        if (slotsB > 0 && (bytes_u8 < 0.97 * bytes_u16 || delta_mode == 2)) {
+            dlt = dalloc<std::uint8_t>(slotsB);
+            first = dalloc<std::uint16_t>(nseg);
+            auto vB = dalloc_bytes(vb * slotsB);
+            IRLBA_HIP_CHECK(hipMemsetAsync(dlt.get(), 0, slotsB, s));
+            IRLBA_HIP_CHECK(hipMemsetAsync(vB.get(), 0, vb * slotsB, s));
+            launch_delta_write(ptr.get(), counts.get(), nseg, fidx.get(), fval.get(), st, ptrB.get(), first.get(),
+                               dlt.get(), vB.get(), s);
+            IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+            ptr = ptrB;
+            fval = vB;
+            fidx.reset();
+            t.slots = slotsB;
+            t.delta = true;
+        }
+    }
     counts.reset();
 
     // nnz-balanced chunks for exactly one resident wave of workgroups: as many
@@ -443,10 +494,16 @@ TiledLines build_lines(irlba_hip_context* ctx, const std::int32_t* d_idx, const 
     t.idx = fidx.get();
     t.val = fval.get();
     t.chunk = chunk.get();
+    t.dlt = dlt.get();
+    t.first = first.get();
     keep.push_back(ptr);
-    keep.push_back(fidx);
+    if (fidx) keep.push_back(fidx);
     keep.push_back(fval);
     keep.push_back(chunk);
+    if (dlt) {
+        keep.push_back(dlt);
+        keep.push_back(first);
+    }
     return t;
 }
 
@@ -1656,6 +1713,24 @@ int irlba_hip_matrix_dims(const irlba_hip_matrix* m, int64_t* rows, int64_t* col
         require(m != nullptr, "null matrix");
         if (rows) *rows = m->base->nr;
         if (cols) *cols = m->base->nc;
+    });
+}
+
+int irlba_hip_sparse_layout(const irlba_hip_matrix* m, int adjoint, irlba_hip_layout* out) {
+    return guarded([&] {
+        require(m && out, "null argument");
+        const auto* sb = dynamic_cast<const SparseBase*>(m->base.get());
+        require(sb != nullptr, "not a sparse matrix handle");
+        const TiledLines& t = adjoint ? sb->by_col : sb->by_row;
+        out->nnz = t.nnz;
+        out->slots = t.slots;
+        out->index_bytes = t.slots * (t.delta ? 1 : 2);
+        out->tile_len = t.tile_len;
+        out->n_tiles = t.n_tiles;
+        out->vec = t.vec;
+        out->lanes_per_line = t.lpl;
+        out->delta = t.delta ? 1 : 0;
+        out->blocks = t.B;
     });
 }
 

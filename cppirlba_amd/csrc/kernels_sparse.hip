@@ -300,6 +300,112 @@ void launch_chunk_bounds(const i64* ptr, i64 nseg, int B, i64* chunk, hipStream_
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
+// ------------------------------------------------------------------ 8-bit delta indices
+// A sorted segment of in-tile indices i_0 < i_1 < ... is stored as one 16-bit
+// first index per segment and one byte per slot: 0 for the first entry, then
+// i_j - i_{j-1}.  A gap above 255 is bridged by value-0 slots of delta 255
+// placed before the entry ("dummies"); padding slots have delta 0.  The indices
+// are recovered by prefix sums over lanes.
+
+// lenB[seg] = entries + dummies of each sorted segment.  One wavefront per segment.
+__global__ void k_delta_count(const i64* __restrict__ ptrA, const unsigned long long* __restrict__ counts, i64 nseg,
+                              const std::uint16_t* __restrict__ idxA, unsigned long long* __restrict__ lenB) {
+    const int lane = threadIdx.x & 63;
+    const i64 wave = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const i64 nwaves = (static_cast<i64>(gridDim.x) * blockDim.x) >> 6;
+    for (i64 seg = wave; seg < nseg; seg += nwaves) {
+        const i64 cnt = static_cast<i64>(counts[seg]), e0 = ptrA[seg];
+        i64 d = 0;
+        for (i64 j = lane + 1; j < cnt; j += 64) d += (static_cast<i64>(idxA[e0 + j]) - idxA[e0 + j - 1] - 1) / 255;
+        d = wave_sum(d);
+        if (lane == 0) lenB[seg] = static_cast<unsigned long long>(cnt + d);
+    }
+}
+
+template <typename VT>
+__global__ void k_delta_write(const i64* __restrict__ ptrA, const unsigned long long* __restrict__ counts, i64 nseg,
+                              const std::uint16_t* __restrict__ idxA, const VT* __restrict__ valA,
+                              const i64* __restrict__ ptrB, std::uint16_t* __restrict__ first,
+                              std::uint8_t* __restrict__ dltB, VT* __restrict__ valB) {
+    const int lane = threadIdx.x & 63;
+    const i64 wave = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const i64 nwaves = (static_cast<i64>(gridDim.x) * blockDim.x) >> 6;
+    for (i64 seg = wave; seg < nseg; seg += nwaves) {
+        const i64 cnt = static_cast<i64>(counts[seg]), e0 = ptrA[seg], o0 = ptrB[seg];
+        if (lane == 0) first[seg] = cnt ? idxA[e0] : 0;
+        i64 run = 0;  // output slots written so far (uniform over the wave)
+        for (i64 jb = 0; jb < cnt; jb += 64) {
+            const i64 j = jb + lane;
+            const bool active = j < cnt;
+            const i64 gap = (active && j > 0) ? static_cast<i64>(idxA[e0 + j]) - idxA[e0 + j - 1] : 0;
+            const i64 d = gap > 0 ? (gap - 1) / 255 : 0;  // dummies before this entry
+            const i64 w = active ? d + 1 : 0;
+            const i64 incl = group_scan_incl<64>(w);
+            if (active) {
+                const i64 pos = o0 + run + incl - w;
+                for (i64 q = 0; q < d; ++q) {
+                    dltB[pos + q] = 255;
+                    valB[pos + q] = VT(0);
+                }
+                dltB[pos + d] = static_cast<std::uint8_t>(gap - 255 * d);
+                valB[pos + d] = valA[e0 + j];
+            }
+            run += __shfl(incl, 63, 64);
+        }
+    }
+}
+
+// idx[s] = first[seg] + (delta prefix through slot s): the 16-bit form, for the
+// kernels that read indices out of order.
+__global__ void k_delta_expand(const i64* __restrict__ ptr, i64 nseg, const std::uint16_t* __restrict__ first,
+                               const std::uint8_t* __restrict__ dlt, std::uint16_t* __restrict__ idx) {
+    const int lane = threadIdx.x & 63;
+    const i64 wave = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const i64 nwaves = (static_cast<i64>(gridDim.x) * blockDim.x) >> 6;
+    for (i64 seg = wave; seg < nseg; seg += nwaves) {
+        const i64 s0 = ptr[seg], s1 = ptr[seg + 1];
+        unsigned base = first[seg];
+        for (i64 sb = s0; sb < s1; sb += 64) {
+            const i64 s = sb + lane;
+            const unsigned d = s < s1 ? dlt[s] : 0u;
+            const unsigned incl = group_scan_incl<64>(d);
+            if (s < s1) idx[s] = static_cast<std::uint16_t>(base + incl);
+            base += __shfl(incl, 63, 64);
+        }
+    }
+}
+
+static i64 per_segment_grid(i64 nseg) { return std::max<i64>(1, std::min<i64>((nseg * 64 + 255) / 256, 65536)); }
+
+void launch_delta_count(const i64* ptrA, const unsigned long long* counts, i64 nseg, const std::uint16_t* idxA,
+                        unsigned long long* lenB, hipStream_t s) {
+    hipLaunchKernelGGL(k_delta_count, dim3(static_cast<unsigned>(per_segment_grid(nseg))), dim3(256), 0, s, ptrA,
+                       counts, nseg, idxA, lenB);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+void launch_delta_write(const i64* ptrA, const unsigned long long* counts, i64 nseg, const std::uint16_t* idxA,
+                        const void* valA, StorageType st, const i64* ptrB, std::uint16_t* first, std::uint8_t* dltB,
+                        void* valB, hipStream_t s) {
+    const dim3 g(static_cast<unsigned>(per_segment_grid(nseg))), b(256);
+    if (st == ST_F64)
+        hipLaunchKernelGGL(k_delta_write<double>, g, b, 0, s, ptrA, counts, nseg, idxA,
+                           static_cast<const double*>(valA), ptrB, first, dltB, static_cast<double*>(valB));
+    else if (st == ST_F32)
+        hipLaunchKernelGGL(k_delta_write<float>, g, b, 0, s, ptrA, counts, nseg, idxA,
+                           static_cast<const float*>(valA), ptrB, first, dltB, static_cast<float*>(valB));
+    else
+        throw std::runtime_error("the delta layout holds FP64 or FP32 values");
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+void launch_delta_expand(const TiledLines& t, std::uint16_t* idx, hipStream_t s) {
+    const i64 nseg = static_cast<i64>(t.n_tiles) * t.n_major;
+    hipLaunchKernelGGL(k_delta_expand, dim3(static_cast<unsigned>(per_segment_grid(nseg))), dim3(256), 0, s, t.ptr,
+                       nseg, t.first, t.dlt, idx);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
 // ------------------------------------------------------------------ tiled SpMV
 template <typename AT>
 struct SpmvParams {
@@ -307,6 +413,8 @@ struct SpmvParams {
     const std::uint16_t* idx;
     const void* val;
     const i64* chunk;
+    const std::uint8_t* dlt;      // 8-bit delta indices (delta layout) ...
+    const std::uint16_t* first;   // ... and the first index of each segment
     i64 n_major, n_minor;
     int tile_len, n_tiles, B;
     Prologue pro;
@@ -348,7 +456,33 @@ struct Loader<VT, AT, 1> {
     }
 };
 
-template <typename VT, typename AT, int LPL, int VEC>
+// Four consecutive values (the delta layout loads its indices separately).
+template <typename VT>
+__device__ __forceinline__ void load_values4(const VT* v, i64 s, VT (&vo)[4]);
+template <>
+__device__ __forceinline__ void load_values4<float>(const float* v, i64 s, float (&vo)[4]) {
+    const f32x4 f = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(v + s));
+    vo[0] = f.x; vo[1] = f.y; vo[2] = f.z; vo[3] = f.w;
+}
+template <>
+__device__ __forceinline__ void load_values4<double>(const double* v, i64 s, double (&vo)[4]) {
+    const f64x2 a = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(v + s));
+    const f64x2 b = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(v + s + 2));
+    vo[0] = a.x; vo[1] = a.y; vo[2] = b.x; vo[3] = b.y;
+}
+
+// In-tile indices of a lane's four slots from their packed deltas: `carry` is
+// the index reached before the group's step, `incl` the group's inclusive scan
+// of the per-lane delta sums `tot`.
+__device__ __forceinline__ void delta_indices(unsigned dw, unsigned carry, unsigned incl, unsigned tot,
+                                              unsigned (&q)[4]) {
+    q[0] = carry + incl - tot + (dw & 0xffu);
+    q[1] = q[0] + ((dw >> 8) & 0xffu);
+    q[2] = q[1] + ((dw >> 16) & 0xffu);
+    q[3] = q[2] + (dw >> 24);
+}
+
+template <typename VT, typename AT, int LPL, int VEC, bool DELTA>
 __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
     extern __shared__ unsigned char smem_raw[];
     AT* xs = reinterpret_cast<AT*>(smem_raw);
@@ -362,6 +496,9 @@ __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
     const int lane_in = threadIdx.x & (LPL - 1);
     const int grp = threadIdx.x / LPL, ngrp = blockDim.x / LPL;
     constexpr int STRIDE = LPL * VEC;
+    // Steps in flight per lane: the tile in LDS allows one 16-wave block per
+    // CU, so up to 128 VGPRs cost no occupancy; FP64 values or sums fit four steps.
+    constexpr int UNR = (sizeof(VT) == 8 || sizeof(AT) == 8) ? 4 : 8;
     const Prologue& pro = p.pro;
     const bool single = p.n_tiles == 1;
     const double coef = (single && p.epi.axpy_src) ? *p.epi.axpy_coef : 0.0;
@@ -397,20 +534,70 @@ __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
 
     // The segment bounds of a group's next line are loaded while the current
     // line streams, so a line never starts with an exposed pointer load.
+    const std::uint16_t* __restrict__ F = DELTA ? p.first + seg0 : nullptr;
     i64 nxt0 = 0, nxt1 = 0;
+    unsigned nxtf = 0;
     if (L0 + grp < L1) {
         nxt0 = P[L0 + grp];
         nxt1 = P[L0 + grp + 1];
+        if constexpr (DELTA) nxtf = F[L0 + grp];
     }
     for (i64 m = L0 + grp; m < L1; m += ngrp) {
         const i64 s0 = nxt0, s1 = nxt1;
+        const unsigned first = nxtf;
         if (m + ngrp < L1) {
             nxt0 = P[m + ngrp];
             nxt1 = P[m + ngrp + 1];
+            if constexpr (DELTA) nxtf = F[m + ngrp];
         }
+        (void)first;
         EpiOperands eop;
         if (single && lane_in == 0) eop = load_epilogue(p.epi, m);  // in flight during the stream
         AT acc0 = 0, acc1 = 0;
+        if constexpr (DELTA) {
+            static_assert(VEC == 4, "the delta layout packs four indices per lane");
+            // Group-uniform iterations of UNR steps: every lane of the group
+            // takes part in each step's scan; lanes past the segment's end load
+            // nothing and add nothing, so the tail costs one iteration, not one
+            // exposed load per step.
+            const std::uint8_t* __restrict__ dl = p.dlt;
+            unsigned carry = first;
+            for (i64 sb = s0; sb < s1; sb += UNR * STRIDE) {
+                VT v[UNR][4];
+                unsigned dw[UNR], tot[UNR], inc[UNR];
+                bool ok[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    const i64 s = sb + u * STRIDE + lane_in * 4;
+                    ok[u] = s < s1;  // segments are padded to whole lanes (4 slots)
+                    dw[u] = 0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) v[u][k] = VT(0);
+                    if (ok[u]) {
+                        load_values4<VT>(val, s, v[u]);
+                        dw[u] = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(dl + s));
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    tot[u] = __builtin_amdgcn_sad_u8(dw[u], 0u, 0u);  // sum of the four bytes
+                    inc[u] = group_scan_incl_dpp<LPL>(tot[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    unsigned q[4];
+                    delta_indices(dw[u], carry, inc[u], tot[u], q);
+                    carry += group_last<LPL>(inc[u]);
+                    if (ok[u]) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            if (u & 1) acc1 += static_cast<AT>(v[u][k]) * xs[q[k]];
+                            else acc0 += static_cast<AT>(v[u][k]) * xs[q[k]];
+                        }
+                    }
+                }
+            }
+        } else {
         i64 s = s0 + lane_in * VEC;
         for (; s + 3 * STRIDE < s1; s += 4 * STRIDE) {
             VT v[4][VEC];
@@ -431,6 +618,7 @@ __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
             Loader<VT, AT, VEC>::load(val, ix, s, v, q);
 #pragma unroll
             for (int k = 0; k < VEC; ++k) acc0 += static_cast<AT>(v[k]) * xs[q[k]];
+        }
         }
         AT acc = group_sum<LPL>(acc0 + acc1);
         if (lane_in == 0) {
@@ -481,6 +669,8 @@ static void spmv_launch_one(const TiledLines& tl, const Prologue& pro, const Epi
     p.idx = tl.idx;
     p.val = tl.val;
     p.chunk = tl.chunk;
+    p.dlt = tl.dlt;
+    p.first = tl.first;
     p.n_major = tl.n_major;
     p.n_minor = tl.n_minor;
     p.tile_len = tl.tile_len;
@@ -492,7 +682,12 @@ static void spmv_launch_one(const TiledLines& tl, const Prologue& pro, const Epi
     p.gamma_part = gamma_part;
     const size_t lds = static_cast<size_t>(tl.tile_len) * sizeof(AT);
     if (ev0) IRLBA_HIP_CHECK(hipEventRecord(ev0, s));
-    hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, VEC>), dim3(tl.B), dim3(1024), lds, s, p);
+    if constexpr (VEC == 4) {
+        if (tl.delta) hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, 4, true>), dim3(tl.B), dim3(1024), lds, s, p);
+        else hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, 4, false>), dim3(tl.B), dim3(1024), lds, s, p);
+    } else {
+        hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, VEC, false>), dim3(tl.B), dim3(1024), lds, s, p);
+    }
     IRLBA_HIP_LAUNCH_CHECK();
     if (ev1) IRLBA_HIP_CHECK(hipEventRecord(ev1, s));
     if (tl.n_tiles > 1) {

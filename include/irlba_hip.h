@@ -111,6 +111,22 @@ int irlba_hip_matrix_destroy(irlba_hip_matrix* m);
 /* Matrix::rows() / cols() (include/irlba/Matrix/interface.hpp:161,166). */
 int irlba_hip_matrix_dims(const irlba_hip_matrix* m, int64_t* rows, int64_t* cols);
 
+/* Device layout of one orientation of a sparse handle (no reference
+ * counterpart: introspection for tests and measurements).  adjoint = 0 is the
+ * row-major layout that A.v streams, 1 the column-major one of A'.u. */
+typedef struct irlba_hip_layout {
+    int64_t nnz;            /* true entries */
+    int64_t slots;          /* stored slots: entries, zero padding, gap-bridging slots */
+    int64_t index_bytes;    /* bytes of the index stream (2 per slot, or 1 with 8-bit deltas) */
+    int32_t tile_len;       /* minor-dimension tile staged in LDS */
+    int32_t n_tiles;
+    int32_t vec;            /* slots per lane per step */
+    int32_t lanes_per_line;
+    int32_t delta;          /* 1: 8-bit delta indices */
+    int32_t blocks;         /* workgroups of one product */
+} irlba_hip_layout;
+int irlba_hip_sparse_layout(const irlba_hip_matrix* m, int adjoint, irlba_hip_layout* out);
+
 /* Workspace::multiply (adjoint = 0) and AdjointWorkspace::multiply (adjoint = 1)
  * (include/irlba/Matrix/interface.hpp:49,87): output is fully overwritten. */
 int irlba_hip_multiply(const irlba_hip_matrix* m, int adjoint, const double* right, double* output,
