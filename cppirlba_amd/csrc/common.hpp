@@ -262,6 +262,8 @@ void launch_normalize(const double* f, i64 L, const double* norm_part, int G, do
 void launch_rotate(const double* M, i64 ld, int work, const double* Yt, int ldy, int k, double* out, i64 ld_out,
                    i64 L, hipStream_t s);
 void launch_scale_copy(const double* src, i64 L, double denom, double* dst, hipStream_t s);
+// dst = src / *denom with the denominator read on the device.
+void launch_scale_copy_dev(const double* src, i64 L, const double* denom, double* dst, hipStream_t s);
 void launch_vec_combine(double* out, const double* a, const double* b, i64 n, int op, hipStream_t s);
 void launch_axpy_dev(double* y, const double* src, const double* coef, i64 n, hipStream_t s);
 void launch_scale_columns(double* M, i64 ld, i64 L, int k, const double* d, hipStream_t s);

@@ -105,9 +105,13 @@ struct irlba_hip_context {
         IRLBA_HIP_CHECK(hipEventCreate(&e));
         return e;
     }
-    // Fold the timings of completed products into the stats (call after a sync).
-    void resolve_events() {
-        for (auto& p : pending) {
+    // Fold the timings of completed products into the stats: all of them
+    // (after a stream sync) or the first `upto` (after waiting on an event
+    // recorded behind them).
+    void resolve_events(size_t upto = static_cast<size_t>(-1)) {
+        upto = std::min(upto, pending.size());
+        for (size_t i = 0; i < upto; ++i) {
+            auto& p = pending[i];
             float ms = 0;
             // A product with an empty dimension launches nothing and records no events.
             if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
@@ -118,7 +122,7 @@ struct irlba_hip_context {
             pool.push_back(p.a);
             pool.push_back(p.b);
         }
-        pending.clear();
+        pending.erase(pending.begin(), pending.begin() + static_cast<std::ptrdiff_t>(upto));
     }
     void sync() {
         IRLBA_HIP_CHECK(hipStreamSynchronize(stream));
@@ -933,6 +937,8 @@ struct Engine {
         ldy = static_cast<int>((work + 15) / 16 * 16);
         Yt = dalloc<double>(2 * work * ldy);
         for (auto& p : yt_pinned) IRLBA_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(double) * work * ldy));
+        IRLBA_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&norms_host), sizeof(double) * (2 * work + 4)));
+        IRLBA_HIP_CHECK(hipEventCreateWithFlags(&norms_ready, hipEventDisableTiming));
         IRLBA_HIP_CHECK(hipMemsetAsync(W.get(), 0, sizeof(double) * nr * work, s));
         IRLBA_HIP_CHECK(hipMemsetAsync(V.get(), 0, sizeof(double) * nc * work, s));
         IRLBA_HIP_CHECK(hipMemsetAsync(dnorm.get(), 0, sizeof(double) * (2 * work + 4), s));
@@ -985,13 +991,29 @@ struct Engine {
             if (h % 2 == 0) phase_a(h / 2); else phase_b(h / 2);
         }
     }
-    void read_norms() {
-        std::vector<double> h(2 * work + 4);
-        IRLBA_HIP_CHECK(hipMemcpyAsync(h.data(), dnorm.get(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
-        ctx->sync();
+    // The next sweep starts from V[:, k] = F / R_F (compute.hpp:280) whatever k
+    // the restart picks, so its first product A V[:, k] can be queued as soon
+    // as the sweep's kernels are: F is scaled in place (R_F read on the
+    // device) and A F lands in W_next.  After an invariant-subspace refill the
+    // sweep recomputes F and W_next, so a queued product is only ever wasted.
+    void queue_next_product() {
+        launch_scale_copy_dev(F.get(), nc, dR(work - 1), F.get(), s);
+        op.apply(false, F.get(), Wn.get(), nullptr, nullptr);
+    }
+    // Copies the norms to the host and waits for that copy only, so work
+    // queued behind it (the speculative first product) keeps the GPU busy
+    // while the host runs the invariant checks and the B-SVD.
+    void read_norms(bool speculate = false) {
+        const i64 n = 2 * work + 4;
+        IRLBA_HIP_CHECK(hipMemcpyAsync(norms_host, dnorm.get(), sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        IRLBA_HIP_CHECK(hipEventRecord(norms_ready, s));
+        const size_t done = ctx->pending.size();
+        if (speculate) queue_next_product();
+        IRLBA_HIP_CHECK(hipEventSynchronize(norms_ready));
+        ctx->resolve_events(done);
         for (i64 j = 0; j < work; ++j) {
-            S[j] = h[j];
-            R[j] = h[work + 1 + j];
+            S[j] = norms_host[j];
+            R[j] = norms_host[work + 1 + j];
         }
     }
     void set_dev_scalar(double* d, double v) {
@@ -1006,10 +1028,12 @@ struct Engine {
     }
 
     // run_lanczos_bidiagonalization (lanczos.hpp:54-143) from column `start`; fills B.
-    void sweep(i64 start, HostMat& B, bool product_ready = false) {
+    // With `speculate`, the next sweep's first product is queued behind every
+    // read of the norms (see queue_next_product).
+    void sweep(i64 start, HostMat& B, bool product_ready = false, bool speculate = false) {
         first(start, product_ready);
         launch_from(2 * start);
-        read_norms();
+        read_norms(speculate);
         if (S[start] < eps) throw std::runtime_error("starting vector near the null space of the input matrix");
         i64 h = 2 * start;
         const i64 hcheck_end = 2 * (work - 1);  // half-steps with an invariant-subspace test
@@ -1044,7 +1068,7 @@ struct Engine {
                 set_dev_scalar(dS(j + 1), 0.0);
             }
             launch_from(ev + 1);
-            read_norms();
+            read_norms(speculate);
             h = ev + 1;
         }
         for (i64 j = start; j < work; ++j) {
@@ -1068,10 +1092,14 @@ struct Engine {
     }
 
     double* yt_pinned[2] = {nullptr, nullptr};
+    double* norms_host = nullptr;  // pinned, so the norms copy is asynchronous
+    hipEvent_t norms_ready = nullptr;
     ~Engine() {
         (void)hipStreamSynchronize(s);
         for (double* p : yt_pinned)
             if (p) (void)hipHostFree(p);
+        if (norms_host) (void)hipHostFree(norms_host);
+        if (norms_ready) (void)hipEventDestroy(norms_ready);
     }
 };
 
@@ -1139,15 +1167,13 @@ std::pair<bool, int> run_compute(irlba_hip_context* ctx, Op& op, i64 number, con
     const double svtol = o.singular_value_ratio_tolerance >= 0 ? o.singular_value_ratio_tolerance : tol;
 
     for (; iter < o.max_iterations; ++iter) {
-        eg.sweep(k, B, iter > 0);
-        // The next sweep starts from V[:, k] = F / R_F (compute.hpp:280) whatever
-        // k the restart picks, so its first product A V[:, k] is queued now and
-        // runs on the GPU while the host computes the SVD of B.  F is scaled in
-        // place and copied into V[:, k] after the V rotation has read the old basis.
-        // (Wasted once, when this restart converges.)
+        // The sweep queues the next sweep's first product A (F / R_F) before it
+        // waits for the norms, so it runs on the GPU while the host checks the
+        // norms and computes the SVD of B; F / R_F is copied into V[:, k] after
+        // the V rotation has read the old basis.  (Wasted once, when this
+        // restart converges.)
+        eg.sweep(k, B, iter > 0, true);
         const double R_F = eg.R[work - 1];
-        launch_scale_copy(eg.F.get(), nc, R_F, eg.F.get(), s);
-        eg.op.apply(false, eg.F.get(), eg.Wn.get(), nullptr, nullptr);
         const auto ts = std::chrono::steady_clock::now();
         host_svd(B, BU, BS, BV);
         st.host_svd_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();

@@ -5,8 +5,9 @@
 //    reduction (deterministic), then f -= M[:, :n] t fused with the per-block
 //    sums of squares of the updated f (the next Lanczos norm).
 //  * Norm finalisation + normalisation (lanczos.hpp:81-85, 99-109, 123-132).
-//  * The restart rotation V[:, :k] <- V Y (compute.hpp:268-280, 302-305) on the
-//    FP64 matrix cores (v_mfma_f64_16x16x4f64), in place per 16-row strip.
+//  * The restart rotation V[:, :k] <- V Y (compute.hpp:268-280, 302-305): on the
+//    vector FP64 units, a thread per row, for long bases with k <= 72, else on
+//    the FP64 matrix cores (v_mfma_f64_16x16x4f64) per 16-row strip; in place.
 #include "common.hpp"
 
 namespace irlba_hip {
@@ -94,10 +95,77 @@ __global__ __launch_bounds__(256) void k_cgs_dots(const double* __restrict__ M, 
     }
 }
 
+// The same product with 16-byte loads: a lane holds two consecutive rows, U
+// row pairs per thread in flight (512-row steps).  Needs an even leading
+// dimension, an even chunk and 16-byte aligned M and f (checked by the
+// launcher); measured ~30% faster than the 8-byte form on a 1e6 x 67 basis.
+template <int C, int U>
+__global__ __launch_bounds__(256) void k_cgs_dots2(const double* __restrict__ M, i64 ld, int n,
+                                                   const double* __restrict__ f, i64 L, i64 chunk,
+                                                   double* __restrict__ partials) {
+    constexpr int span = 64 / C;
+    __shared__ double red[4][C];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const i64 r0 = static_cast<i64>(blockIdx.x) * chunk;
+    const i64 r1 = min(L, r0 + chunk);
+    for (int cb = blockIdx.y * C; cb < n; cb += C * gridDim.y) {
+        double v[C];
+#pragma unroll
+        for (int q = 0; q < C; ++q) v[q] = 0;
+        const int nb = min(C, n - cb);
+        for (i64 r = r0 + 2 * threadIdx.x; r < r1; r += 512 * U) {
+            // r is even and L is even, so r < r1 also covers row r + 1.
+            f64x2 fr[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                fr[u] = r + 512 * u < r1 ? *reinterpret_cast<const f64x2*>(f + r + 512 * u) : f64x2{0, 0};
+            const double* p = M + r + static_cast<i64>(cb) * ld;
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                if (q < nb) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (r + 512 * u < r1) {
+                            const f64x2 m = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(p + 512 * u));
+                            v[q] += m[0] * fr[u][0] + m[1] * fr[u][1];
+                        }
+                }
+                p += ld;
+            }
+        }
+#pragma unroll
+        for (int half = C / 2, o = 32; half >= 1; half >>= 1, o >>= 1) {
+            const bool upper = (lane & o) != 0;
+#pragma unroll
+            for (int j = 0; j < half; ++j) {
+                const double send = upper ? v[j] : v[j + half];
+                const double keep = upper ? v[j + half] : v[j];
+                v[j] = keep + __shfl_xor(send, o, 64);
+            }
+        }
+#pragma unroll
+        for (int o = span / 2; o >= 1; o >>= 1) v[0] += __shfl_xor(v[0], o, 64);
+        if ((lane & (span - 1)) == 0) red[w][lane / span] = v[0];
+        __syncthreads();
+        if (threadIdx.x < nb)
+            partials[static_cast<i64>(blockIdx.x) * n + cb + threadIdx.x] =
+                (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+        __syncthreads();
+    }
+}
+
+static inline bool pairs_ok(const double* a, const double* b, i64 ld, i64 L) {
+    return (ld % 2 == 0) && (L % 2 == 0) && (reinterpret_cast<std::uintptr_t>(a) % 16 == 0) &&
+           (reinterpret_cast<std::uintptr_t>(b) % 16 == 0);
+}
+
 void launch_cgs_dots(const double* M, i64 ld, int n, const double* f, i64 L, double* partials, hipStream_t s) {
     if (n <= 0) return;
     const int G = cgs_dot_blocks(L);
-    if (L >= kDotLong)
+    if (L >= kDotLong && pairs_ok(M, f, ld, L)) {
+        const i64 chunk = (cgs_chunk(L, G) + 1) & ~static_cast<i64>(1);
+        hipLaunchKernelGGL((k_cgs_dots2<8, 2>), dim3(G, (n + 7) / 8), dim3(256), 0, s, M, ld, n, f, L, chunk, partials);
+    } else if (L >= kDotLong)
         hipLaunchKernelGGL((k_cgs_dots<8, 4>), dim3(G, (n + 7) / 8), dim3(256), 0, s, M, ld, n, f, L, cgs_chunk(L, G),
                            partials);
     else
@@ -179,13 +247,81 @@ __global__ __launch_bounds__(256) void k_cgs_update(const double* __restrict__ M
     if (threadIdx.x == 0) norm_part[blockIdx.x] = sq;
 }
 
+// The same update with 16-byte accesses (a lane holds two consecutive rows, U
+// pairs per thread, CB columns per batch); the conditions of k_cgs_dots2 apply.
+template <int U, int CB>
+__global__ __launch_bounds__(256) void k_cgs_update2(const double* __restrict__ M, i64 ld, int n,
+                                                     const double* __restrict__ t, double* __restrict__ f, i64 L,
+                                                     i64 chunk, double* __restrict__ norm_part) {
+    extern __shared__ double ts[];
+    __shared__ double red[32];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) ts[i] = t[i];
+    __syncthreads();
+    const i64 r0 = static_cast<i64>(blockIdx.x) * chunk;
+    const i64 r1 = min(L, r0 + chunk);
+    double sq = 0;
+    for (i64 r = r0 + 2 * threadIdx.x; r < r1; r += 512 * U) {
+        bool in[U];
+        f64x2 acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            in[u] = r + 512 * u < r1;
+            acc[u] = f64x2{0, 0};
+        }
+        const double* p = M + r;
+        int i = 0;
+        for (; i + CB <= n; i += CB) {
+            f64x2 m[CB][U];
+#pragma unroll
+            for (int c = 0; c < CB; ++c)
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    m[c][u] = in[u] ? __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(p + c * ld + 512 * u))
+                                    : f64x2{0, 0};
+#pragma unroll
+            for (int c = 0; c < CB; ++c)
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    acc[u][0] += m[c][u][0] * ts[i + c];
+                    acc[u][1] += m[c][u][1] * ts[i + c];
+                }
+            p += CB * ld;
+        }
+        for (; i < n; ++i) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (in[u]) {
+                    const f64x2 m = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(p + 512 * u));
+                    acc[u][0] += m[0] * ts[i];
+                    acc[u][1] += m[1] * ts[i];
+                }
+            p += ld;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (in[u]) {
+                f64x2 v = *reinterpret_cast<const f64x2*>(f + r + 512 * u);
+                v[0] -= acc[u][0];
+                v[1] -= acc[u][1];
+                *reinterpret_cast<f64x2*>(f + r + 512 * u) = v;
+                sq += v[0] * v[0] + v[1] * v[1];
+            }
+    }
+    double dummy = 0;
+    block_sum2(sq, dummy, red);
+    if (threadIdx.x == 0) norm_part[blockIdx.x] = sq;
+}
+
 void launch_cgs_update(const double* M, i64 ld, int n, const double* t, double* f, i64 L, double* norm_part,
                        hipStream_t s) {
     const int G = cgs_blocks(L);  // the norm finalisation sums exactly these partials
     const size_t lds = sizeof(double) * static_cast<size_t>(std::max(n, 1));
     if (lds > 60000) throw std::runtime_error("working subspace too large for the CGS update kernel");
     const i64 chunk = cgs_chunk(L, G);
-    if (chunk >= 512)
+    if (chunk >= 512 && pairs_ok(M, f, ld, L)) {
+        const i64 chunk2 = (chunk + 1) & ~static_cast<i64>(1);  // G * chunk2 >= L; trailing blocks write 0
+        hipLaunchKernelGGL((k_cgs_update2<2, 8>), dim3(G), dim3(256), lds, s, M, ld, n, t, f, L, chunk2, norm_part);
+    } else if (chunk >= 512)
         hipLaunchKernelGGL((k_cgs_update<4, 4>), dim3(G), dim3(256), lds, s, M, ld, n, t, f, L, chunk, norm_part);
     else
         hipLaunchKernelGGL((k_cgs_update<1, 16>), dim3(G), dim3(256), lds, s, M, ld, n, t, f, L, chunk, norm_part);
@@ -243,6 +379,20 @@ __global__ void k_scale_copy(const double* __restrict__ src, i64 L, double denom
 void launch_scale_copy(const double* src, i64 L, double denom, double* dst, hipStream_t s) {
     const i64 grid = std::max<i64>(1, std::min<i64>((L + 255) / 256, 4096));
     hipLaunchKernelGGL(k_scale_copy, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, src, L, denom, dst);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+__global__ void k_scale_copy_dev(const double* __restrict__ src, i64 L, const double* __restrict__ denom,
+                                 double* __restrict__ dst) {
+    const double d = *denom;
+    for (i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; r < L;
+         r += static_cast<i64>(gridDim.x) * blockDim.x)
+        dst[r] = src[r] / d;
+}
+
+void launch_scale_copy_dev(const double* src, i64 L, const double* denom, double* dst, hipStream_t s) {
+    const i64 grid = std::max<i64>(1, std::min<i64>((L + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_scale_copy_dev, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, src, L, denom, dst);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
@@ -408,9 +558,78 @@ static void rotate_launch(const double* M, i64 ld, int work, const double* Yt, i
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
+// The same rotation on the vector FP64 units: a thread per row keeps NK
+// accumulators, streams its row of M (CB columns of loads in flight; lanes read
+// consecutive rows, so every load is coalesced) and takes Y row by row as
+// wave-uniform scalar loads.  It computes exactly k (rounded up to 8) columns
+// instead of the 16-column MFMA tiles, and measured 303 us against 426 us for
+// the MFMA form on a 1e6 x 70 basis with k = 66 (that shape sits at the FP64
+// ridge point, ~8.5 flop per byte).  A thread reads its whole row before it
+// writes, so out == M is safe.
+template <int NK, int CB>
+__global__ __launch_bounds__(256) void k_rotate_rows(const double* M, i64 ld, int work, const double* __restrict__ Yt,
+                                                     int ldy, int k, double* out, i64 ld_out, i64 L) {
+    const i64 r = static_cast<i64>(blockIdx.x) * 256 + threadIdx.x;
+    if (r >= L) return;
+    double acc[NK];
+#pragma unroll
+    for (int j = 0; j < NK; ++j) acc[j] = 0;
+    const double* p = M + r;
+    int c = 0;
+    for (; c + CB <= work; c += CB) {
+        double m[CB];
+#pragma unroll
+        for (int q = 0; q < CB; ++q) m[q] = __builtin_nontemporal_load(p + static_cast<i64>(c + q) * ld);
+#pragma unroll
+        for (int q = 0; q < CB; ++q) {
+            const double* y = Yt + static_cast<i64>(c + q) * ldy;
+#pragma unroll
+            for (int j = 0; j < NK; ++j) acc[j] = __builtin_fma(m[q], y[j], acc[j]);
+        }
+    }
+    for (; c < work; ++c) {
+        const double mv = __builtin_nontemporal_load(p + static_cast<i64>(c) * ld);
+        const double* y = Yt + static_cast<i64>(c) * ldy;
+#pragma unroll
+        for (int j = 0; j < NK; ++j) acc[j] = __builtin_fma(mv, y[j], acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < NK; ++j)
+        if (j < k) __builtin_nontemporal_store(acc[j], out + r + static_cast<i64>(j) * ld_out);
+}
+
+template <int NK>
+static void rotate_rows_launch(const double* M, i64 ld, int work, const double* Yt, int ldy, int k, double* out,
+                               i64 ld_out, i64 L, hipStream_t s) {
+    const i64 grid = (L + 255) / 256;
+    hipLaunchKernelGGL((k_rotate_rows<NK, 5>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, M, ld, work, Yt,
+                       ldy, k, out, ld_out, L);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+// Bases at least this long with k <= 72 take the vector-FMA rotation (enough
+// threads to fill the GPU); shorter ones and wider k take the MFMA tiles.
+constexpr i64 kRotateRowsMin = 65536;
+
 void launch_rotate(const double* M, i64 ld, int work, const double* Yt, int ldy, int k, double* out, i64 ld_out,
                    i64 L, hipStream_t s) {
     if (k <= 0 || L <= 0) return;
+    if (L >= kRotateRowsMin && k <= 72 && (k + 7) / 8 * 8 <= ldy) {
+        // Y columns between k and the next multiple of 8 are read (ldy covers
+        // them) and their sums discarded.
+        switch ((k + 7) / 8) {
+            case 1: rotate_rows_launch<8>(M, ld, work, Yt, ldy, k, out, ld_out, L, s); break;
+            case 2: rotate_rows_launch<16>(M, ld, work, Yt, ldy, k, out, ld_out, L, s); break;
+            case 3: rotate_rows_launch<24>(M, ld, work, Yt, ldy, k, out, ld_out, L, s); break;
+            case 4: rotate_rows_launch<32>(M, ld, work, Yt, ldy, k, out, ld_out, L, s); break;
+            case 5: rotate_rows_launch<40>(M, ld, work, Yt, ldy, k, out, ld_out, L, s); break;
+            case 6: rotate_rows_launch<48>(M, ld, work, Yt, ldy, k, out, ld_out, L, s); break;
+            case 7: rotate_rows_launch<56>(M, ld, work, Yt, ldy, k, out, ld_out, L, s); break;
+            case 8: rotate_rows_launch<64>(M, ld, work, Yt, ldy, k, out, ld_out, L, s); break;
+            default: rotate_rows_launch<72>(M, ld, work, Yt, ldy, k, out, ld_out, L, s); break;
+        }
+        return;
+    }
     if (out == M && k > 128) throw std::runtime_error("in-place rotation covers at most 128 columns per pass");
     for (int cb = 0; cb < k; cb += 128) {
         const int nct = std::min(8, (k - cb + 15) / 16);

@@ -12,9 +12,11 @@ pytestmark = pytest.mark.gpu
 irlba = pytest.importorskip("cppirlba_amd")
 
 
-# (300007, 45, 45): long vectors take the 4-rows-per-thread dot and update kernels.
+# (300007, 45, 45): long odd-length vectors take the 4-rows-per-thread dot and update kernels;
+# (262144, 13, 11) and (600002, 37, 36) even ones the 16-byte (row pair) dots, and the latter
+# also the row-pair update.
 @pytest.mark.parametrize("rows,cols,n", [(5, 4, 4), (1000, 70, 33), (100003, 40, 40), (300, 9, 0), (300007, 45, 45),
-                                         (262144, 13, 11)])
+                                         (262144, 13, 11), (600002, 37, 36)])
 def test_orthogonalize_vs_oracle(rows, cols, n):
     rng = np.random.default_rng(rows + n)
     M, _ = np.linalg.qr(rng.standard_normal((rows, cols)))
@@ -27,8 +29,10 @@ def test_orthogonalize_vs_oracle(rows, cols, n):
         assert np.abs(M[:, :n].T @ out).max() < 1e-12
 
 
+# Bases of 65536+ rows with k <= 72 take the thread-per-row FMA rotation, the rest the MFMA tiles.
 @pytest.mark.parametrize("rows,work,k", [(16, 4, 1), (64, 8, 5), (1000, 70, 67), (777, 27, 20), (50001, 33, 30),
-                                         (300, 200, 150)])
+                                         (300, 200, 150), (65536, 9, 3), (200001, 70, 66), (131073, 80, 72),
+                                         (70000, 90, 80)])
 def test_rotation_vs_numpy(rows, work, k):
     rng = np.random.default_rng(rows * 7 + k)
     M = rng.standard_normal((rows, work))

@@ -317,6 +317,24 @@ def test_pca_sparse_vs_oracle_and_stats():
     assert np.all(np.abs(res.scores.sum(axis=0)) < 1e-8)
 
 
+def test_pca_long_even_basis_vs_oracle():
+    # 600002 columns: the V basis is long and of even length, so its Gram-Schmidt
+    # runs on the 16-byte (row pair) dot and update kernels and its restart
+    # rotation on the thread-per-row FMA kernel; same start vector as the oracle.
+    nr, nc = 200, 600002
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.02, 31, f32=False)
+    init = O.create_random_vector(nc, 9)
+    opts = dict(extra_work=10, convergence_tolerance=1e-6)
+    H = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr)
+    res = irlba.pca(H, True, True, 5, irlba.Options(initial=init, **opts))
+    sc, rot, var, conv, it = O.pca(O.sparse(nr, nc, vals, idx, ptr, True, 4), True, True, 5, initial=init, **opts)
+    assert res.converged and conv
+    assert_equal_vectors(res.variances, var, 1e-8)
+    assert subspace_angle(res.rotation, rot) < 1e-5
+    assert np.abs(res.rotation.T @ res.rotation - np.eye(5)).max() < 1e-10
+    assert np.all(np.abs(res.scores.sum(axis=0)) < 1e-8)
+
+
 def test_pca_full_size_fp64_and_mixed():
     # 30000 x 100000 at 5% (1.5e8 nnz), FP64 storage, pca top-50 with centring and
     # scaling, extra_work 20: too large for the oracle, so size-independent checks.
