@@ -123,8 +123,9 @@ class Context:
         check(lib().irlba_hip_get_stats(self.handle, ctypes.byref(s)))
         return {name: getattr(s, name) for name, _ in _lib.Stats._fields_}
 
-    def set_profiling(self, enabled):
-        check(lib().irlba_hip_context_set_profiling(self.handle, int(enabled)))
+    def set_profiling(self, period):
+        """Time one product in `period` (1 = all, 0 = none) with HIP events."""
+        check(lib().irlba_hip_context_set_profiling(self.handle, int(period)))
 
     @property
     def stream(self):

@@ -44,6 +44,8 @@ class Stats(ctypes.Structure):
         ("host_svd_ms", ctypes.c_double),
         ("restarts", ctypes.c_int32),
         ("refills", ctypes.c_int32),
+        ("forward_timed", ctypes.c_int64),
+        ("adjoint_timed", ctypes.c_int64),
     ]
 
 

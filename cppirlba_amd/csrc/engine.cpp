@@ -86,7 +86,8 @@ struct irlba_hip_context {
     int cu_count = 256;
     size_t lds_per_block = 65536;
     size_t lds_per_cu = 65536;
-    bool profiling = true;
+    int profiling = 1;     // time one product in `profiling` per direction (0: none)
+    i64 profile_seq[2] = {0, 0};
     irlba_hip_stats stats{};
     struct Pending {
         hipEvent_t a, b;
@@ -299,10 +300,11 @@ void apply_handle(const irlba_hip_matrix& m, bool adjoint, const double* x, doub
     }
     irlba_hip_context* ctx = m.ctx;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (ctx->profiling) {
+    if (ctx->profiling > 0 && ctx->profile_seq[adjoint ? 1 : 0]++ % ctx->profiling == 0) {
         e0 = ctx->get_event();
         e1 = ctx->get_event();
         ctx->pending.push_back({e0, e1, adjoint});
+        if (adjoint) ++ctx->stats.adjoint_timed; else ++ctx->stats.forward_timed;
     }
     if (adjoint) ++ctx->stats.adjoint_products; else ++ctx->stats.forward_products;
     m.base->product(adjoint, pro, epi, e0, e1);
@@ -936,7 +938,7 @@ struct Engine {
         norm_part = dalloc<double>(G);
         ldy = static_cast<int>((work + 15) / 16 * 16);
         Yt = dalloc<double>(2 * work * ldy);
-        for (auto& p : yt_pinned) IRLBA_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(double) * work * ldy));
+        IRLBA_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&yt_host), sizeof(double) * 2 * work * ldy));
         IRLBA_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&norms_host), sizeof(double) * (2 * work + 4)));
         IRLBA_HIP_CHECK(hipEventCreateWithFlags(&norms_ready, hipEventDisableTiming));
         IRLBA_HIP_CHECK(hipMemsetAsync(W.get(), 0, sizeof(double) * nr * work, s));
@@ -1077,27 +1079,33 @@ struct Engine {
         }
     }
 
-    // out[:, :k] = M[:, :work] Y[:, :k] on the matrix cores (out == M allowed).
-    // `slot` picks one of two pinned staging buffers, so the V and W rotations
-    // of one restart queue without a host round trip; a slot is rewritten only
-    // after the next sweep's synchronisation.
-    void rotate(int slot, double* M, i64 L, const HostMat& Y, i64 k, double* out, i64 ld_out) {
-        double* yt = yt_pinned[slot];
+    // Rotation coefficients Y[:, :k] go row-major (zero padded to ldy columns)
+    // into one of two slots of a pinned staging buffer; upload_y copies slots
+    // [0, n) in one transfer, so the V and W rotations of a restart cost one
+    // host-to-device copy.  A slot is rewritten only after the next sweep's
+    // synchronisation (or an explicit one).
+    void stage_y(int slot, const HostMat& Y, i64 k) {
+        double* yt = yt_host + static_cast<i64>(slot) * work * ldy;
         std::fill(yt, yt + work * ldy, 0.0);
         for (i64 r = 0; r < work; ++r)
             for (i64 c = 0; c < k; ++c) yt[r * ldy + c] = Y(r, c);
-        double* dY = Yt.get() + static_cast<i64>(slot) * work * ldy;
-        IRLBA_HIP_CHECK(hipMemcpyAsync(dY, yt, sizeof(double) * work * ldy, hipMemcpyHostToDevice, s));
+    }
+    void upload_y(int nslots) {
+        IRLBA_HIP_CHECK(hipMemcpyAsync(Yt.get(), yt_host, sizeof(double) * nslots * work * ldy, hipMemcpyHostToDevice,
+                                       s));
+    }
+    // out[:, :k] = M[:, :work] Y[:, :k] with the staged slot's Y (out == M allowed).
+    void rotate_staged(int slot, double* M, i64 L, i64 k, double* out, i64 ld_out) {
+        const double* dY = Yt.get() + static_cast<i64>(slot) * work * ldy;
         launch_rotate(M, L, static_cast<int>(work), dY, ldy, static_cast<int>(k), out, ld_out, L, s);
     }
 
-    double* yt_pinned[2] = {nullptr, nullptr};
+    double* yt_host = nullptr;
     double* norms_host = nullptr;  // pinned, so the norms copy is asynchronous
     hipEvent_t norms_ready = nullptr;
     ~Engine() {
         (void)hipStreamSynchronize(s);
-        for (double* p : yt_pinned)
-            if (p) (void)hipHostFree(p);
+        if (yt_host) (void)hipHostFree(yt_host);
         if (norms_host) (void)hipHostFree(norms_host);
         if (norms_ready) (void)hipEventDestroy(norms_ready);
     }
@@ -1200,9 +1208,12 @@ std::pair<bool, int> run_compute(irlba_hip_context* ctx, Op& op, i64 number, con
         }
         prevS = BS;
         k = update_k(k, requested, n_conv, work);
-        eg.rotate(0, eg.V.get(), nc, BV, k, eg.V.get(), nc);
+        eg.stage_y(0, BV, k);
+        eg.stage_y(1, BU, k);
+        eg.upload_y(2);
+        eg.rotate_staged(0, eg.V.get(), nc, k, eg.V.get(), nc);
         IRLBA_HIP_CHECK(hipMemcpyAsync(eg.Vc(k), eg.F.get(), sizeof(double) * nc, hipMemcpyDeviceToDevice, s));
-        eg.rotate(1, eg.W.get(), nr, BU, k, eg.W.get(), nr);
+        eg.rotate_staged(1, eg.W.get(), nr, k, eg.W.get(), nr);
         std::fill(B.a.begin(), B.a.end(), 0.0);
         for (i64 l = 0; l < k; ++l) {
             B(l, l) = BS[l];
@@ -1229,8 +1240,11 @@ std::pair<bool, int> run_compute(irlba_hip_context* ctx, Op& op, i64 number, con
         // After max_iterations the last restart's rotations may still be reading
         // the pinned staging slots; finish them before the slots are rewritten.
         IRLBA_HIP_CHECK(hipStreamSynchronize(s));
-        eg.rotate(0, eg.W.get(), nr, BU, requested, oU, nr);
-        eg.rotate(1, eg.V.get(), nc, BV, requested, oV, nc);
+        eg.stage_y(0, BU, requested);
+        eg.stage_y(1, BV, requested);
+        eg.upload_y(2);
+        eg.rotate_staged(0, eg.W.get(), nr, requested, oU, nr);
+        eg.rotate_staged(1, eg.V.get(), nc, requested, oV, nc);
     }
     const auto kind = device_ptrs ? hipMemcpyHostToDevice : hipMemcpyHostToHost;
     IRLBA_HIP_CHECK(hipMemcpyAsync(D, BS.data(), sizeof(double) * requested, kind, s));
@@ -1626,8 +1640,12 @@ int irlba_hip_get_stats(const irlba_hip_context* ctx, irlba_hip_stats* out) {
     return guarded([&] { *out = ctx->stats; });
 }
 
-int irlba_hip_context_set_profiling(irlba_hip_context* ctx, int enabled) {
-    return guarded([&] { ctx->profiling = enabled != 0; });
+int irlba_hip_context_set_profiling(irlba_hip_context* ctx, int period) {
+    return guarded([&] {
+        if (period < 0) throw std::runtime_error("profiling period must be non-negative");
+        ctx->profiling = period;
+        ctx->profile_seq[0] = ctx->profile_seq[1] = 0;
+    });
 }
 
 void* irlba_hip_context_stream(irlba_hip_context* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }

@@ -615,8 +615,17 @@ void launch_rotate(const double* M, i64 ld, int work, const double* Yt, int ldy,
                    i64 L, hipStream_t s) {
     if (k <= 0 || L <= 0) return;
     if (L >= kRotateRowsMin && k <= 72 && (k + 7) / 8 * 8 <= ldy) {
-        // Y columns between k and the next multiple of 8 are read (ldy covers
-        // them) and their sums discarded.
+        // Y columns between k and the accumulator count (the next multiple of
+        // 8, or of 2 from 65 to 68, where the restarts of a 70-vector subspace
+        // sit) are read (ldy covers them) and their sums discarded.
+        if (k > 64 && k <= 66) {
+            rotate_rows_launch<66>(M, ld, work, Yt, ldy, k, out, ld_out, L, s);
+            return;
+        }
+        if (k > 66 && k <= 68) {
+            rotate_rows_launch<68>(M, ld, work, Yt, ldy, k, out, ld_out, L, s);
+            return;
+        }
         switch ((k + 7) / 8) {
             case 1: rotate_rows_launch<8>(M, ld, work, Yt, ldy, k, out, ld_out, L, s); break;
             case 2: rotate_rows_launch<16>(M, ld, work, Yt, ldy, k, out, ld_out, L, s); break;

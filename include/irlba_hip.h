@@ -63,6 +63,8 @@ typedef struct irlba_hip_stats {
     double host_svd_ms;           /* time in the host FP64 SVD of B */
     int32_t restarts;
     int32_t refills;              /* invariant-subspace refills (lanczos.hpp:101-106,124-130) */
+    int64_t forward_timed;        /* A.v products whose main kernel was timed (see set_profiling) */
+    int64_t adjoint_timed;
 } irlba_hip_stats;
 
 /* Options defaults of Options.hpp:27-82. */
@@ -77,8 +79,12 @@ int irlba_hip_context_create(int device, irlba_hip_context** out);
 int irlba_hip_context_destroy(irlba_hip_context* ctx);
 int irlba_hip_context_synchronize(irlba_hip_context* ctx);
 int irlba_hip_get_stats(const irlba_hip_context* ctx, irlba_hip_stats* out);
-/* 1 = time every product's main kernel with HIP events (default 1). */
-int irlba_hip_context_set_profiling(irlba_hip_context* ctx, int enabled);
+/* Time the main kernel of one product in `period` of each direction (A.v and
+ * A'.u counted apart) with HIP events (1 = every product, the default; 0 =
+ * none).  Each timed product adds two event records to the stream, a few
+ * microseconds of idle GPU around the kernel, so long solves are timed with a
+ * sampling period such as 8. */
+int irlba_hip_context_set_profiling(irlba_hip_context* ctx, int period);
 /* The stream the context launches on (a hipStream_t), for callers timing it. */
 void* irlba_hip_context_stream(irlba_hip_context* ctx);
 

@@ -31,7 +31,7 @@ def test_orthogonalize_vs_oracle(rows, cols, n):
 
 # Bases of 65536+ rows with k <= 72 take the thread-per-row FMA rotation, the rest the MFMA tiles.
 @pytest.mark.parametrize("rows,work,k", [(16, 4, 1), (64, 8, 5), (1000, 70, 67), (777, 27, 20), (50001, 33, 30),
-                                         (300, 200, 150), (65536, 9, 3), (200001, 70, 66), (131073, 80, 72),
+                                         (300, 200, 150), (65536, 9, 3), (200001, 70, 66), (70001, 70, 67), (131073, 80, 72),
                                          (70000, 90, 80)])
 def test_rotation_vs_numpy(rows, work, k):
     rng = np.random.default_rng(rows * 7 + k)
