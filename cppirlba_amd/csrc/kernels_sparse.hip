@@ -482,13 +482,24 @@ __device__ __forceinline__ void delta_indices(unsigned dw, unsigned carry, unsig
     q[3] = q[2] + (dw >> 24);
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b to XCD b % 8),
+// and each XCD has its own L2.  Giving XCD x the consecutive chunks
+// [x B/8, (x+1) B/8) keeps the blocks that stage the same input tile on one
+// XCD, so a tile is fetched from HBM about once per product instead of once
+// per block (A.v of the 31-tile 30000 x 1e6 matrix: ~190 MB less traffic).
+__device__ __forceinline__ int xcd_chunk(int b, int nb) {
+    constexpr int kXcds = 8;
+    return (nb % kXcds == 0) ? (b % kXcds) * (nb / kXcds) + b / kXcds : b;
+}
+
 template <typename VT, typename AT, int LPL, int VEC, bool DELTA>
 __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
     extern __shared__ unsigned char smem_raw[];
     AT* xs = reinterpret_cast<AT*>(smem_raw);
     __shared__ double red[2 * 16];
 
-    const i64 S0 = p.chunk[blockIdx.x], S1 = p.chunk[blockIdx.x + 1];
+    const int cid = xcd_chunk(blockIdx.x, gridDim.x);
+    const i64 S0 = p.chunk[cid], S1 = p.chunk[cid + 1];
     if (S0 >= S1) return;  // uniform across the block
     const i64 t_first = S0 / p.n_major, t_last = (S1 - 1) / p.n_major;
     const VT* __restrict__ val = static_cast<const VT*>(p.val);
