@@ -374,8 +374,9 @@ def _run_compute(matrix, number, options, ctx=None):
     else:
         raise TypeError("compute() takes a device matrix handle or an irlba Matrix subclass")
     k = nout.value
-    return Results(U[: nr * k].reshape(k, nr).T.copy(), V[: nc * k].reshape(k, nc).T.copy(), D[:k].copy(),
-                   it.value, bool(conv.value))
+    # Column-major (Fortran-ordered) views of the engine's output buffers, as the
+    # reference's Eigen matrices are laid out; no host transpose.
+    return Results(U[: nr * k].reshape(k, nr).T, V[: nc * k].reshape(k, nc).T, D[:k], it.value, bool(conv.value))
 
 
 def compute(matrix, number, options=None):
@@ -397,28 +398,41 @@ def column_stats(matrix, center=True, scale=True):
     return (c if center else None), (s if scale else None)
 
 
-def pca(matrix, center, scale, number, options=None, storage="f64", ctx=None):
+def pca(matrix, center, scale, number, options=None, storage="f64", ctx=None, device=False):
     """pca(matrix, center, scale, number, options) -> PcaResults (pca.hpp:55-139, 197-204).
 
-    ``matrix`` is a device handle, or a dense array (wrapped like SimpleMatrix)."""
+    ``matrix`` is a device handle, or a dense array (wrapped like SimpleMatrix).
+    The results are column-major numpy arrays, or with ``device=True`` float64
+    torch tensors left in the GPU's memory (no transfer over PCIe)."""
     if not isinstance(matrix, _Handle):
         matrix = HipDenseMatrix(matrix, storage=storage, ctx=ctx)
     options = options or Options()
     o, keep = options._to_c()
     nr, nc = matrix.shape
     n = max(int(number), 0)
-    S = np.zeros(nr * n)
-    R = np.zeros(nc * n)
-    var = np.zeros(max(n, 1))
+    if device:
+        import torch
+
+        dev = torch.device("cuda", matrix.ctx.device)
+        S = torch.empty(max(n, 1) * nr, dtype=torch.float64, device=dev)
+        R = torch.empty(max(n, 1) * nc, dtype=torch.float64, device=dev)
+        var = torch.empty(max(n, 1), dtype=torch.float64, device=dev)
+        torch.cuda.synchronize(dev)  # the engine writes them on its own stream
+        ptrs = (S.data_ptr(), R.data_ptr(), var.data_ptr())
+    else:
+        S = np.zeros(nr * n)
+        R = np.zeros(nc * n)
+        var = np.zeros(max(n, 1))
+        ptrs = (S.ctypes.data, R.ctypes.data, var.ctypes.data)
     nout = ctypes.c_int64()
     conv = ctypes.c_int32()
     it = ctypes.c_int32()
-    check(lib().irlba_hip_pca(matrix.handle, int(center), int(scale), number, ctypes.byref(o), S.ctypes.data,
-                              R.ctypes.data, var.ctypes.data, ctypes.byref(nout), ctypes.byref(conv),
-                              ctypes.byref(it), 0))
+    check(lib().irlba_hip_pca(matrix.handle, int(center), int(scale), number, ctypes.byref(o), *ptrs,
+                              ctypes.byref(nout), ctypes.byref(conv), ctypes.byref(it), int(bool(device))))
     k = nout.value
-    return PcaResults(S[: nr * k].reshape(k, nr).T.copy(), R[: nc * k].reshape(k, nc).T.copy(), var[:k].copy(),
-                      it.value, bool(conv.value))
+    # Column-major views (pca.hpp:154-160 lays them out as Eigen matrices): no transpose copy.
+    return PcaResults(S[: nr * k].reshape(k, nr).T, R[: nc * k].reshape(k, nc).T, var[:k], it.value,
+                      bool(conv.value))
 
 
 def generate_sparse(nrow, ncol, density, seed, storage="f32", accumulate=None, ctx=None):

@@ -315,6 +315,14 @@ def test_pca_sparse_vs_oracle_and_stats():
     assert_equal_vectors(res.variances, var, 1e-8)
     assert subspace_angle(res.rotation, rot) < 1e-5
     assert np.all(np.abs(res.scores.sum(axis=0)) < 1e-8)
+    # Results left in device memory: the same run (fixed-order reductions), bit for bit.
+    dres = irlba.pca(H, True, True, 10, irlba.Options(extra_work=20, initial=init, convergence_tolerance=1e-8),
+                     device=True)
+    assert dres.rotation.is_cuda and tuple(dres.rotation.shape) == (nc, 10) and tuple(dres.scores.shape) == (nr, 10)
+    assert dres.iterations == res.iterations
+    np.testing.assert_array_equal(dres.variances.cpu().numpy(), res.variances)
+    np.testing.assert_array_equal(dres.rotation.cpu().numpy(), res.rotation)
+    np.testing.assert_array_equal(dres.scores.cpu().numpy(), res.scores)
 
 
 def test_pca_long_even_basis_vs_oracle():
