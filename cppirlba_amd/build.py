@@ -1,7 +1,8 @@
 """Builds the HIP engine into cppirlba_amd/lib/libirlba_hip.so (gfx950).
 
-hipcc compiles every source of csrc/ to an object in _build/ (in parallel)
-and links one shared library.  All HIP code is compiled with packed FP32
+hipcc compiles every HIP source of csrc/ to an object in _build/ (in
+parallel), g++ the host-only sources (the host SVD, built for two x86 ISA
+levels with run-time dispatch), and hipcc links one shared library.  All HIP code is compiled with packed FP32
 instructions disabled (-target-feature -packed-fp32-ops).  The libtorch
 autograd binding (csrc/torch_svd.cpp, host code only) is compiled with g++
 against torch's headers into lib/libirlba_torch.so, linked to the engine.
@@ -32,6 +33,10 @@ SOURCES = [
     "kernels_spmm.hip",
 ]
 
+# Host-only C++ (no device code), compiled by g++.
+HOST_SOURCES = ["host_svd.cpp"]
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra"]
+
 COMMON = [
     "-O3",
     "-std=c++17",
@@ -52,10 +57,13 @@ def hipcc():
 
 def _compile(src):
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *COMMON]
-    if src.endswith(".cpp"):
-        cmd += ["-x", "hip"]
-    cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
+    if src in HOST_SOURCES:
+        cmd = ["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = [hipcc(), *COMMON]
+        if src.endswith(".cpp"):
+            cmd += ["-x", "hip"]
+        cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr}")
@@ -66,7 +74,7 @@ def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
-        results = list(ex.map(_compile, SOURCES))
+        results = list(ex.map(_compile, SOURCES + HOST_SOURCES))
     for obj, err in results:
         if verbose and err.strip():
             sys.stderr.write(err)
