@@ -131,7 +131,10 @@ void host_svd(const HostMat& A, HostMat& U, std::vector<double>& S, HostMat& V) 
         host_svd(T, V, S, U);
         return;
     }
-    static const bool fast = __builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma");
+    static const bool fast = [] {
+        __builtin_cpu_init();
+        return __builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma");
+    }();
     if (fast) jacobi_svd_avx2(A, U, S, V);
     else jacobi_svd_base(A, U, S, V);
 }
