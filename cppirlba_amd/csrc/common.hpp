@@ -272,8 +272,10 @@ void launch_scale_columns(double* M, i64 ld, i64 L, int k, const double* d, hipS
 // Row stride of the packed SDDMM factors for rank R (FP32 or FP64 factors).
 int sddmm_padded_rank(int R, bool f32, int* lanes_per_entry = nullptr);
 void launch_pack_rows(const double* X, i64 rows, i64 ld, int R, int Rp, bool f32, void* out, hipStream_t s);
-void launch_sddmm(const std::int32_t* idx, const i64* ptr, i64 n_lines, const void* line_rm, const void* entry_rm,
-                  int R, bool f32, void* out, hipStream_t s);
+// out[e] = line_rm[line] . entry_rm[idx[e]] over a compressed pattern, checking
+// the layout as it reads it (bit 0 of *flag: pointers, bit 1: indices).
+void launch_sddmm(const std::int32_t* idx, const i64* ptr, i64 n_lines, i64 n_other, const void* line_rm,
+                  const void* entry_rm, int R, bool f32, void* out, int* flag, hipStream_t s);
 int gram_blocks(i64 L);
 // out[i * ld_out + j] = A[:, i] . B[:, j] for L x k operands (k <= 64); partials
 // holds gram_blocks(L) * k * k doubles.
@@ -285,6 +287,9 @@ void launch_coldots(const double* A, i64 lda, const double* B, i64 ldb, int k, i
 void launch_cols_axpby(double* out, i64 ldo, const double* a_coef, double a_scale, const double* x, i64 ldx,
                        const double* b_coef, double b_scale, int k, i64 L, hipStream_t s);
 void launch_cg_coefs(const double* a, double* b, double* out, int k, int mode, hipStream_t s);
+void launch_backward_coefs(const double* J, const double* K, int ldy, const double* sv, const double* w, int k,
+                           int complement, double* CX, double* CY, double* d, double* sig2, int* flag,
+                           hipStream_t s);
 void launch_axpby(double* out, const double* a, double alpha, const double* b, double beta, i64 n, hipStream_t s);
 void launch_fill(double* out, double v, i64 n, hipStream_t s);
 

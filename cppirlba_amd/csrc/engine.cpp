@@ -135,6 +135,13 @@ struct irlba_hip_context {
     // buffer serves them all.
     std::shared_ptr<void> ws;
     size_t ws_bytes = 0;
+    // A device int cleared in stream order, for the error bits of checking kernels.
+    std::shared_ptr<int> flag_buf;
+    int* flag() {
+        if (!flag_buf) flag_buf = dalloc<int>(1);
+        IRLBA_HIP_CHECK(hipMemsetAsync(flag_buf.get(), 0, sizeof(int), stream));
+        return flag_buf.get();
+    }
     char* workspace(size_t bytes) { return grow(ws, ws_bytes, bytes); }
     // A second one for the multi-vector products, which run inside calls that
     // hold the first.
@@ -1284,7 +1291,7 @@ int run_svd_backward(const irlba_hip_matrix* m, i64 k, const double* U, const do
         p.push_back(c.take<double>(static_cast<i64>(std::max(gb, cb)) * std::max<i64>(k * k, R)));  // 6 partials
         p.push_back(c.take<double>(k * ldy));      // 7 J (rotation layout)
         p.push_back(c.take<double>(k * ldy));      // 8 K
-        p.push_back(c.take<double>(2 * k * ldy + 2 * k));  // 9 coefficient staging: CX, CY, d, sig2
+        p.push_back(c.take<double>(2 * k * ldy + 4 * k));  // 9 coefficients CX, CY, d, sig2 (+ staged s, w)
         p.push_back(dcenter ? c.take<double>(nr + R + nc) : nullptr);  // 10 ones, column sums, centre gradient
         if (exact) {
             for (int side = 0; side < 2; ++side) {
@@ -1320,47 +1327,29 @@ int run_svd_backward(const irlba_hip_matrix* m, i64 k, const double* U, const do
     }
     double* partials = wp[6];
     double *dJ = wp[7], *dK = wp[8], *coef = wp[9];
-    std::vector<double> hs(k), hw(k);
-    if (device_ptrs) {
-        IRLBA_HIP_CHECK(hipMemcpyAsync(hs.data(), sv, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-        IRLBA_HIP_CHECK(hipMemcpyAsync(hw.data(), w, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-    } else {
-        std::copy(sv, sv + k, hs.begin());
-        std::copy(w, w + k, hw.begin());
+    double *dCX = coef, *dCY = coef + k * ldy, *dd = dCY + k * ldy, *dsig2 = dd + k;
+    const double *dsv = sv, *dw = w;
+    if (!device_ptrs) {  // stage s and w behind the coefficients
+        double* st = dsig2 + k;
+        IRLBA_HIP_CHECK(hipMemcpyAsync(st, sv, sizeof(double) * k, hipMemcpyHostToDevice, s));
+        IRLBA_HIP_CHECK(hipMemcpyAsync(st + k, w, sizeof(double) * k, hipMemcpyHostToDevice, s));
+        dsv = st;
+        dw = st + k;
     }
-    // J = U'G_U, K = V'G_V, laid out as rotation coefficients (J(i, j) at i * ldy + j).
+    // J = U'G_U, K = V'G_V, laid out as rotation coefficients (J(i, j) at i * ldy + j);
+    // the coefficients follow on the device (no host round trip); the check of
+    // the singular values is read back with the results.
+    int* flag = ctx->flag();
     launch_gram(dU, nr, dGU, nr, ki, nr, partials, dJ, ldy, s);
     launch_gram(dV, nc, dGV, nc, ki, nc, partials, dK, ldy, s);
-    std::vector<double> J(k * ldy), K(k * ldy);
-    IRLBA_HIP_CHECK(hipMemcpyAsync(J.data(), dJ, sizeof(double) * k * ldy, hipMemcpyDeviceToHost, s));
-    IRLBA_HIP_CHECK(hipMemcpyAsync(K.data(), dK, sizeof(double) * k * ldy, hipMemcpyDeviceToHost, s));
-    IRLBA_HIP_CHECK(hipStreamSynchronize(s));
-    for (i64 i = 0; i < k; ++i)
-        require(hs[i] > 0 && (i == 0 || hs[i] <= hs[i - 1]), "singular values must be positive and non-increasing");
-
-    // Host: CX = M (- J S^-1 in mode 1), CY = -K S^-1 (mode 1), d = 1 / s, sig2 = s^2.
-    std::vector<double> hc(static_cast<size_t>(2 * k * ldy + 2 * k), 0.0);
-    double *CX = hc.data(), *CY = hc.data() + k * ldy, *hd = CY + k * ldy, *hsig2 = hd + k;
-    for (i64 i = 0; i < k; ++i)
-        for (i64 j = 0; j < k; ++j) {
-            double mij = 0;
-            if (i == j) {
-                mij = hw[i];
-            } else {
-                const double den = hs[j] * hs[j] - hs[i] * hs[i];
-                const double F = den != 0 ? 1.0 / den : 0.0;  // repeated values: undefined gradient, term dropped
-                const double ju = J[i * ldy + j] - J[j * ldy + i], kv = K[i * ldy + j] - K[j * ldy + i];
-                mij = F * ju * hs[j] + hs[i] * F * kv;
-            }
-            CX[i * ldy + j] = mij - (complement == 1 ? J[i * ldy + j] / hs[j] : 0.0);
-            CY[i * ldy + j] = complement == 1 ? -K[i * ldy + j] / hs[j] : 0.0;
-        }
-    for (i64 i = 0; i < k; ++i) {
-        hd[i] = 1.0 / hs[i];
-        hsig2[i] = hs[i] * hs[i];
-    }
-    IRLBA_HIP_CHECK(hipMemcpyAsync(coef, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice, s));
-    const double *dCX = coef, *dCY = coef + k * ldy, *dd = dCY + k * ldy, *dsig2 = dd + k;
+    launch_backward_coefs(dJ, dK, ldy, dsv, dw, ki, complement, dCX, dCY, dd, dsig2, flag, s);
+    auto check_values = [&] {
+        int hflag = 0;
+        IRLBA_HIP_CHECK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+        require(!(hflag & 1), "singular values must be positive and non-increasing");
+    };
+    if (exact) check_values();  // before the CG, which would not converge on them
 
     // X = [U | U CX (+ G_U S^-1)], Y = [Y' | V].
     double *X0 = Xo, *X1 = Xo + nr * k, *Y0 = Yo, *Y1 = Yo + nc * k;
@@ -1450,7 +1439,7 @@ int run_svd_backward(const irlba_hip_matrix* m, i64 k, const double* U, const do
         IRLBA_HIP_CHECK(hipMemcpyAsync(Y, Yo, sizeof(double) * nc * R, hipMemcpyDeviceToHost, s));
         if (dcenter) IRLBA_HIP_CHECK(hipMemcpyAsync(dcenter, dc, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
     }
-    IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+    check_values();  // synchronises
     return iterations;
 }
 
@@ -2022,23 +2011,28 @@ int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const in
         const bool f32 = out_type == IRLBA_HIP_F32;
         const size_t ob = f32 ? 4 : 8;
         const i64 n_lines = column_major ? ncol : nrow, n_other = column_major ? nrow : ncol;
-        i64 nnz = 0;
-        IRLBA_HIP_CHECK(hipMemcpyAsync(&nnz, pointers + n_lines, sizeof(i64),
-                                       device_pointers ? hipMemcpyDeviceToHost : hipMemcpyHostToHost, s));
-        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
-        require(nnz >= 0, "pointers are not a valid compressed layout");
-        require(nnz == 0 || (indices && out), "null argument");
+        // The entry count is needed on the host to stage host arrays, or to
+        // accept null index/output pointers for an empty pattern; device arrays
+        // are otherwise used as they are, their layout checked by the kernel.
+        i64 nnz = -1;
+        if (!device_pointers || !indices || !out) {
+            IRLBA_HIP_CHECK(hipMemcpyAsync(&nnz, pointers + n_lines, sizeof(i64),
+                                           device_pointers ? hipMemcpyDeviceToHost : hipMemcpyHostToHost, s));
+            IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+            require(nnz >= 0, "pointers are not a valid compressed layout");
+            require(nnz == 0 || (indices && out), "null argument");
+        }
         const int Rp = sddmm_padded_rank(static_cast<int>(rank), f32);
         // Device arrays are used in place; host ones are staged in the workspace.
+        const i64 stage_nnz = device_pointers ? 0 : nnz;
         auto plan = [&](Carve& c, std::vector<void*>& p) {
             p.clear();
-            p.push_back(c.take<int>(1));                                           // 0 flag
-            p.push_back(c.take<char>(static_cast<i64>(ob) * Rp * (nrow + ncol)));  // 1 packed factors
-            p.push_back(device_pointers ? nullptr : c.take<i64>(n_lines + 1));     // 2 pointers
-            p.push_back(device_pointers ? nullptr : c.take<std::int32_t>(nnz));    // 3 indices
-            p.push_back(device_pointers ? nullptr : c.take<double>(nrow * rank));  // 4 X
-            p.push_back(device_pointers ? nullptr : c.take<double>(ncol * rank));  // 5 Y
-            p.push_back(device_pointers ? nullptr : c.take<char>(static_cast<i64>(ob) * nnz));  // 6 out
+            p.push_back(c.take<char>(static_cast<i64>(ob) * Rp * (nrow + ncol)));  // 0 packed factors
+            p.push_back(device_pointers ? nullptr : c.take<i64>(n_lines + 1));     // 1 pointers
+            p.push_back(device_pointers ? nullptr : c.take<std::int32_t>(stage_nnz));  // 2 indices
+            p.push_back(device_pointers ? nullptr : c.take<double>(nrow * rank));  // 3 X
+            p.push_back(device_pointers ? nullptr : c.take<double>(ncol * rank));  // 4 Y
+            p.push_back(device_pointers ? nullptr : c.take<char>(static_cast<i64>(ob) * stage_nnz));  // 5 out
         };
         std::vector<void*> wp;
         Carve sizing;
@@ -2051,38 +2045,38 @@ int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const in
         const double *dX = X, *dY = Y;
         void* dout = out;
         if (!device_pointers) {
-            IRLBA_HIP_CHECK(hipMemcpyAsync(wp[2], pointers, sizeof(i64) * (n_lines + 1), hipMemcpyHostToDevice, s));
+            IRLBA_HIP_CHECK(hipMemcpyAsync(wp[1], pointers, sizeof(i64) * (n_lines + 1), hipMemcpyHostToDevice, s));
             if (nnz)
                 IRLBA_HIP_CHECK(
-                    hipMemcpyAsync(wp[3], indices, sizeof(std::int32_t) * nnz, hipMemcpyHostToDevice, s));
-            IRLBA_HIP_CHECK(hipMemcpyAsync(wp[4], X, sizeof(double) * nrow * rank, hipMemcpyHostToDevice, s));
-            IRLBA_HIP_CHECK(hipMemcpyAsync(wp[5], Y, sizeof(double) * ncol * rank, hipMemcpyHostToDevice, s));
-            dptr = static_cast<const i64*>(wp[2]);
-            didx = static_cast<const std::int32_t*>(wp[3]);
-            dX = static_cast<const double*>(wp[4]);
-            dY = static_cast<const double*>(wp[5]);
-            dout = wp[6];
+                    hipMemcpyAsync(wp[2], indices, sizeof(std::int32_t) * nnz, hipMemcpyHostToDevice, s));
+            IRLBA_HIP_CHECK(hipMemcpyAsync(wp[3], X, sizeof(double) * nrow * rank, hipMemcpyHostToDevice, s));
+            IRLBA_HIP_CHECK(hipMemcpyAsync(wp[4], Y, sizeof(double) * ncol * rank, hipMemcpyHostToDevice, s));
+            dptr = static_cast<const i64*>(wp[1]);
+            didx = static_cast<const std::int32_t*>(wp[2]);
+            dX = static_cast<const double*>(wp[3]);
+            dY = static_cast<const double*>(wp[4]);
+            dout = wp[5];
         }
-        // The same checks as matrix creation: monotone pointers, indices in range.
-        int* flag = static_cast<int*>(wp[0]);
-        IRLBA_HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
-        launch_validate_compressed(didx, dptr, n_lines, n_other, nnz, flag, s);
+        // Both factors packed row-major (stride Rp) in the output precision.
+        char* packed = static_cast<char*>(wp[0]);
+        char* pX = packed;
+        char* pY = packed + ob * Rp * nrow;
+        int* flag = ctx->flag();
+        launch_pack_rows(dX, nrow, nrow, static_cast<int>(rank), Rp, f32, pX, s);
+        launch_pack_rows(dY, ncol, ncol, static_cast<int>(rank), Rp, f32, pY, s);
+        if (nnz != 0)
+            launch_sddmm(didx, dptr, n_lines, n_other, column_major ? pY : pX, column_major ? pX : pY,
+                         static_cast<int>(rank), f32, dout, flag, s);
         int hflag = 0;
         IRLBA_HIP_CHECK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
         IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+        // The same checks as matrix creation: monotone pointers, indices in range.
         require(!(hflag & 1), "pointers are not a valid compressed layout");
         require(!(hflag & 2), "indices out of range");
-        // Both factors packed row-major (stride Rp) in the output precision.
-        char* packed = static_cast<char*>(wp[1]);
-        char* pX = packed;
-        char* pY = packed + ob * Rp * nrow;
-        launch_pack_rows(dX, nrow, nrow, static_cast<int>(rank), Rp, f32, pX, s);
-        launch_pack_rows(dY, ncol, ncol, static_cast<int>(rank), Rp, f32, pY, s);
-        launch_sddmm(didx, dptr, n_lines, column_major ? pY : pX, column_major ? pX : pY, static_cast<int>(rank), f32,
-                     dout, s);
-        if (!device_pointers && nnz)
+        if (!device_pointers && nnz > 0) {
             IRLBA_HIP_CHECK(hipMemcpyAsync(out, dout, ob * nnz, hipMemcpyDeviceToHost, s));
-        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+            IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+        }
     });
 }
 

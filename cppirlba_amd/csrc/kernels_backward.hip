@@ -57,25 +57,38 @@ struct Piece<double> {
     static constexpr int n = 2;
 };
 
+// The compressed layout is checked as it is read (the checks of matrix
+// creation): a line whose bounds are not monotone within [0, ptr[n_lines]], or
+// ptr[0] != 0, sets bit 0 of *flag and is skipped; an index outside
+// [0, n_other) sets bit 1 and its output is 0.
 template <typename FT, int LPE>
 __global__ __launch_bounds__(256) void k_sddmm(const std::int32_t* __restrict__ idx, const i64* __restrict__ ptr,
-                                               i64 n_lines, const FT* __restrict__ line_rm,
-                                               const FT* __restrict__ entry_rm, int Rp, FT* __restrict__ out) {
+                                               i64 n_lines, i64 n_other, const FT* __restrict__ line_rm,
+                                               const FT* __restrict__ entry_rm, int Rp, FT* __restrict__ out,
+                                               int* __restrict__ flag) {
     typedef typename Piece<FT>::type PT;
     constexpr int VN = Piece<FT>::n;
     constexpr int G = kWave / LPE;  // entries in flight per wavefront
     const int lane = threadIdx.x & 63, q = lane % LPE, g = lane / LPE;
     const i64 wave = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const i64 nwaves = (static_cast<i64>(gridDim.x) * blockDim.x) >> 6;
+    const i64 nnz = ptr[n_lines];
+    if (wave == 0 && lane == 0 && ptr[0] != 0) atomicOr(flag, 1);
     for (i64 c = wave; c < n_lines; c += nwaves) {
         const PT yl = *reinterpret_cast<const PT*>(line_rm + c * Rp + q * VN);
         const i64 e0 = ptr[c], e1 = ptr[c + 1];
+        if (!(0 <= e0 && e0 <= e1 && e1 <= nnz)) {  // uniform across the wavefront
+            if (lane == 0) atomicOr(flag, 1);
+            continue;
+        }
         // Every lane runs the same trip count so the group shuffles stay converged.
         for (i64 base = e0; base < e1; base += G) {
             const i64 e = base + g;
             double acc = 0;
-            if (e < e1) {
-                const i64 row = idx[e];
+            const i64 row = e < e1 ? idx[e] : 0;
+            const bool in = row >= 0 && row < n_other;
+            if (e < e1 && !in && q == 0) atomicOr(flag, 2);
+            if (e < e1 && in) {
                 // A normal (cached) load: each factor row is shared by every entry of its row.
                 const PT x = *reinterpret_cast<const PT*>(entry_rm + row * Rp + q * VN);
 #pragma unroll
@@ -88,11 +101,12 @@ __global__ __launch_bounds__(256) void k_sddmm(const std::int32_t* __restrict__ 
 }
 
 template <typename FT, int LPE>
-static void sddmm_launch(const std::int32_t* idx, const i64* ptr, i64 n_lines, const void* line_rm,
-                         const void* entry_rm, int Rp, void* out, hipStream_t s) {
+static void sddmm_launch(const std::int32_t* idx, const i64* ptr, i64 n_lines, i64 n_other, const void* line_rm,
+                         const void* entry_rm, int Rp, void* out, int* flag, hipStream_t s) {
     const i64 grid = std::max<i64>(1, std::min<i64>((n_lines + 3) / 4, 1 << 20));
     hipLaunchKernelGGL((k_sddmm<FT, LPE>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, idx, ptr, n_lines,
-                       static_cast<const FT*>(line_rm), static_cast<const FT*>(entry_rm), Rp, static_cast<FT*>(out));
+                       n_other, static_cast<const FT*>(line_rm), static_cast<const FT*>(entry_rm), Rp,
+                       static_cast<FT*>(out), flag);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
@@ -105,30 +119,30 @@ int sddmm_padded_rank(int R, bool f32, int* lpe_out) {
     return lpe * vn;
 }
 
-void launch_sddmm(const std::int32_t* idx, const i64* ptr, i64 n_lines, const void* line_rm, const void* entry_rm,
-                  int R, bool f32, void* out, hipStream_t s) {
+void launch_sddmm(const std::int32_t* idx, const i64* ptr, i64 n_lines, i64 n_other, const void* line_rm,
+                  const void* entry_rm, int R, bool f32, void* out, int* flag, hipStream_t s) {
     if (n_lines <= 0) return;
     int lpe = 1;
     const int Rp = sddmm_padded_rank(R, f32, &lpe);
     if (f32) {
         switch (lpe) {
-            case 1: sddmm_launch<float, 1>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
-            case 2: sddmm_launch<float, 2>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
-            case 4: sddmm_launch<float, 4>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
-            case 8: sddmm_launch<float, 8>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
-            case 16: sddmm_launch<float, 16>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
-            case 32: sddmm_launch<float, 32>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
-            default: sddmm_launch<float, 64>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+            case 1: sddmm_launch<float, 1>(idx, ptr, n_lines, n_other, line_rm, entry_rm, Rp, out, flag, s); break;
+            case 2: sddmm_launch<float, 2>(idx, ptr, n_lines, n_other, line_rm, entry_rm, Rp, out, flag, s); break;
+            case 4: sddmm_launch<float, 4>(idx, ptr, n_lines, n_other, line_rm, entry_rm, Rp, out, flag, s); break;
+            case 8: sddmm_launch<float, 8>(idx, ptr, n_lines, n_other, line_rm, entry_rm, Rp, out, flag, s); break;
+            case 16: sddmm_launch<float, 16>(idx, ptr, n_lines, n_other, line_rm, entry_rm, Rp, out, flag, s); break;
+            case 32: sddmm_launch<float, 32>(idx, ptr, n_lines, n_other, line_rm, entry_rm, Rp, out, flag, s); break;
+            default: sddmm_launch<float, 64>(idx, ptr, n_lines, n_other, line_rm, entry_rm, Rp, out, flag, s); break;
         }
     } else {
         switch (lpe) {
-            case 1: sddmm_launch<double, 1>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
-            case 2: sddmm_launch<double, 2>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
-            case 4: sddmm_launch<double, 4>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
-            case 8: sddmm_launch<double, 8>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
-            case 16: sddmm_launch<double, 16>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
-            case 32: sddmm_launch<double, 32>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
-            default: sddmm_launch<double, 64>(idx, ptr, n_lines, line_rm, entry_rm, Rp, out, s); break;
+            case 1: sddmm_launch<double, 1>(idx, ptr, n_lines, n_other, line_rm, entry_rm, Rp, out, flag, s); break;
+            case 2: sddmm_launch<double, 2>(idx, ptr, n_lines, n_other, line_rm, entry_rm, Rp, out, flag, s); break;
+            case 4: sddmm_launch<double, 4>(idx, ptr, n_lines, n_other, line_rm, entry_rm, Rp, out, flag, s); break;
+            case 8: sddmm_launch<double, 8>(idx, ptr, n_lines, n_other, line_rm, entry_rm, Rp, out, flag, s); break;
+            case 16: sddmm_launch<double, 16>(idx, ptr, n_lines, n_other, line_rm, entry_rm, Rp, out, flag, s); break;
+            case 32: sddmm_launch<double, 32>(idx, ptr, n_lines, n_other, line_rm, entry_rm, Rp, out, flag, s); break;
+            default: sddmm_launch<double, 64>(idx, ptr, n_lines, n_other, line_rm, entry_rm, Rp, out, flag, s); break;
         }
     }
 }
@@ -271,6 +285,53 @@ void launch_cols_axpby(double* out, i64 ldo, const double* a_coef, double a_scal
     const i64 grid = std::max<i64>(1, std::min<i64>((L * k + 255) / 256, 8192));
     hipLaunchKernelGGL(k_cols_axpby, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, out, ldo, a_coef, a_scale, x,
                        ldx, b_coef, b_scale, k, L);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+// The k x k coefficients of the backward from J = U'G_U, K = V'G_V (rotation
+// layout, J(i, j) at i * ldy + j), the singular values s and their weights w:
+//   M_ij = w_i (i == j),  F_ij (J_ij - J_ji) s_j + s_i F_ij (K_ij - K_ji) else,
+//   F_ij = 1 / (s_j^2 - s_i^2) (0 for repeated values: the term is dropped);
+//   CX = M - J S^-1 and CY = -K S^-1 with the rank-k complement (mode 1), else
+//   CX = M, CY = 0; d = 1 / s, sig2 = s^2.  Columns k..ldy-1 are zeroed.  Sets
+//   bit 0 of *flag unless s is positive and non-increasing.  One block.
+__global__ void k_backward_coefs(const double* __restrict__ J, const double* __restrict__ K, int ldy,
+                                 const double* __restrict__ sv, const double* __restrict__ w, int k, int complement,
+                                 double* __restrict__ CX, double* __restrict__ CY, double* __restrict__ d,
+                                 double* __restrict__ sig2, int* __restrict__ flag) {
+    for (int e = threadIdx.x; e < k * ldy; e += blockDim.x) {
+        const int i = e / ldy, j = e - i * ldy;
+        double cx = 0, cy = 0;
+        if (j < k) {
+            const double si = sv[i], sj = sv[j];
+            double mij;
+            if (i == j) {
+                mij = w[i];
+            } else {
+                const double den = sj * sj - si * si;
+                const double F = den != 0 ? 1.0 / den : 0.0;
+                const double ju = J[i * ldy + j] - J[j * ldy + i], kv = K[i * ldy + j] - K[j * ldy + i];
+                mij = F * ju * sj + si * F * kv;
+            }
+            cx = mij - (complement == 1 ? J[i * ldy + j] / sj : 0.0);
+            cy = complement == 1 ? -K[i * ldy + j] / sj : 0.0;
+        }
+        CX[e] = cx;
+        CY[e] = cy;
+    }
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        const double si = sv[i];
+        d[i] = 1.0 / si;
+        sig2[i] = si * si;
+        if (!(si > 0) || (i > 0 && !(si <= sv[i - 1]))) atomicOr(flag, 1);
+    }
+}
+
+void launch_backward_coefs(const double* J, const double* K, int ldy, const double* sv, const double* w, int k,
+                           int complement, double* CX, double* CY, double* d, double* sig2, int* flag,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_backward_coefs, dim3(1), dim3(256), 0, s, J, K, ldy, sv, w, k, complement, CX, CY, d, sig2,
+                       flag);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 

@@ -142,7 +142,8 @@ Factored factored_grad(AutogradContext* ctx, const variable_list& g, size_t cent
     const int complement = static_cast<int>(ctx->saved_data["complement"].toInt());
     const bool want_center = ctx->saved_data["center"].toBool() && ctx->needs_input_grad(center_edge);
     const int64_t nr = Ucm.size(1), nc = Vcm.size(1);
-    if (dev >= 0) torch::cuda::synchronize(dev);
+    // The incoming gradients and their conversions are ordered on torch's
+    // stream; one synchronisation after them hands them to the engine's stream.
     at::Tensor gU = g[0].defined() ? col_major(g[0], dev) : at::Tensor();
     at::Tensor gS = g[1].defined() ? g[1].to(f64_on(dev).device(), at::kDouble).contiguous() : at::Tensor();
     at::Tensor gV = g[2].defined() ? col_major(g[2], dev) : at::Tensor();

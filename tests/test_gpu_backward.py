@@ -146,6 +146,47 @@ def test_sddmm_rank_bounds_and_bad_pattern():
     assert out.size == 0
 
 
+def test_sddmm_device_pointers_and_checked_layout():
+    # Device arrays are read in place, their layout checked by the kernel as it
+    # streams them: the same results as the host route, and the same errors.
+    import ctypes
+
+    import torch
+
+    from cppirlba_amd import _lib
+
+    nr, nc, R = 300, 200, 8
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.05, 4, f32=False)
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((nr, R))
+    Y = rng.standard_normal((nc, R))
+    host = irlba.sddmm(nr, nc, idx, ptr, X, Y)
+    ctx = irlba.default_context()
+
+    def dev_sddmm(i, p):
+        ti = torch.tensor(i, dtype=torch.int32, device="cuda")
+        tp = torch.tensor(p, dtype=torch.int64, device="cuda")
+        tX = torch.tensor(np.asfortranarray(X).T.copy().reshape(-1), device="cuda")  # column-major nr x R
+        tY = torch.tensor(np.asfortranarray(Y).T.copy().reshape(-1), device="cuda")
+        out = torch.zeros(len(i), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        _lib.check(_lib.lib().irlba_hip_sddmm(ctx.handle, nr, nc, ctypes.c_void_p(ti.data_ptr()),
+                                              ctypes.c_void_p(tp.data_ptr()), 1, R, ctypes.c_void_p(tX.data_ptr()),
+                                              ctypes.c_void_p(tY.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                              _lib.F64, 1))
+        return out.cpu().numpy()
+
+    np.testing.assert_array_equal(dev_sddmm(idx, ptr), host)
+    bad_idx = idx.copy()
+    bad_idx[len(bad_idx) // 2] = nr  # one row past the end
+    with pytest.raises(irlba.HipError, match="out of range"):
+        dev_sddmm(bad_idx, ptr)
+    bad_ptr = ptr.copy()
+    bad_ptr[5], bad_ptr[6] = ptr[6], ptr[5] - 1  # a decreasing pair of bounds
+    with pytest.raises(irlba.HipError, match="compressed layout"):
+        dev_sddmm(idx, bad_ptr)
+
+
 def test_centered_backward_and_center_gradient():
     nr, nc, k = 300, 120, 5
     vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.1, 21, f32=False)
