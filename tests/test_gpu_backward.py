@@ -146,6 +146,33 @@ def test_sddmm_rank_bounds_and_bad_pattern():
     assert out.size == 0
 
 
+@pytest.mark.parametrize("column_major", [True, False])
+@pytest.mark.parametrize("out_type", ["f64", "f32"])
+def test_sddmm_large_entry_factors(column_major, out_type):
+    # Entry factors larger than an XCD's share of L2 (20000 x 32) and lines
+    # whose indices are not sorted.
+    R, n_other, lines = 32, 20000, 300
+    nr, nc = (n_other, lines) if column_major else (lines, n_other)
+    vals, idx, ptr = O.gen_nb_sparse(n_other, lines, 0.02, 3, f32=False)
+    rng = np.random.default_rng(11)
+    for c in range(0, lines, 7):  # shuffle some lines
+        rng.shuffle(idx[ptr[c]:ptr[c + 1]])
+    X = rng.standard_normal((nr, R))
+    Y = rng.standard_normal((nc, R))
+    owner = np.repeat(np.arange(lines), np.diff(ptr))
+    rows, cols = (idx, owner) if column_major else (owner, idx)
+    ref = np.einsum("ij,ij->i", X[rows], Y[cols])
+    out = irlba.sddmm(nr, nc, idx, ptr, X, Y, column_major=column_major, out_type=out_type)
+    if out_type == "f64":
+        assert np.abs(out - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+    else:
+        assert np.abs(out - ref).max() <= 1e-6 * np.sqrt(R) * np.abs(X).max() * np.abs(Y).max() * 4
+    bad = idx.copy()
+    bad[len(bad) // 3] = n_other + 5
+    with pytest.raises(irlba.HipError, match="out of range"):
+        irlba.sddmm(nr, nc, bad, ptr, X, Y, column_major=column_major, out_type=out_type)
+
+
 def test_sddmm_device_pointers_and_checked_layout():
     # Device arrays are read in place, their layout checked by the kernel as it
     # streams them: the same results as the host route, and the same errors.
