@@ -329,8 +329,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: the IRLBA solve is not sharded across GPUs in this build "
-              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: the IRLBA solve is not sharded across GPUs (a column-sharded "
+              "layout with one all-reduce per forward product is the design, not built); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     if args.mode == "spmv":
         spmv_mode(args)

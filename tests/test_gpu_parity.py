@@ -262,7 +262,7 @@ def test_invariant_subspace_refills(rank):
     np.testing.assert_allclose(res.U @ np.diag(res.D) @ res.V.T, A, atol=1e-8)
 
 
-def test_cfg1_planted_dense_vs_oracle():
+def test_planted_dense_1000x200_vs_oracle():
     # Dense 1000x200 FP64 = U diag(10*0.8^i) V' + 0.01 N(0,1); top-5, extra_work=20,
     # max_iterations=50, tol=1e-5, explicit start vector (seed 1234).
     rng = np.random.default_rng(20240601)
