@@ -231,7 +231,9 @@ int irlba_hip_svd_backward(const irlba_hip_matrix* m, int64_t k, const double* U
  * and Y ncol x rank column-major FP64: the gradient of the stored values of a
  * sparse matrix from the factored dL/dA of irlba_hip_svd_backward.  `out_type`
  * F64 or F32 (the values' type): F32 output rounds the factors to FP32 and
- * accumulates each dot product in FP64.  rank <= 128. */
+ * accumulates each dot product in FP64.  rank <= 128.  The pattern is checked
+ * as the kernel reads it (monotone pointers from 0, indices in range); a bad
+ * pattern fails the call after the kernel, its offending entries unwritten or 0. */
 int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_t* indices,
                     const int64_t* pointers, int column_major, int64_t rank, const double* X, const double* Y,
                     void* out, int out_type, int device_pointers);
