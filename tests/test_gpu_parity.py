@@ -417,3 +417,38 @@ def test_callback_matrix_matches_device_matrix():
     r2 = irlba.compute(irlba.HipDenseMatrix(A), 6, opt)
     assert_equal_vectors(r1.D, r2.D, 1e-10)
     assert_equal_column_vectors(r1.U, r2.U, 1e-6)
+
+
+def test_sampled_product_timing_and_device_results():
+    # Context.set_profiling(period): one product in `period` of each direction
+    # carries HIP events; the counts and the results do not depend on it.
+    ctx = irlba.default_context()
+    A = O.create_random_matrix(300, 120)
+    init = O.create_random_vector(120, 4)
+    opt = irlba.Options(initial=init, extra_work=10)
+    H = irlba.HipDenseMatrix(A)
+    runs = {}
+    try:
+        for period in (1, 3, 0):
+            ctx.set_profiling(period)
+            r = irlba.compute(H, 4, opt)
+            st = ctx.stats()
+            runs[period] = (r, st)
+            for side in ("forward", "adjoint"):
+                n, timed = st[f"{side}_products"], st[f"{side}_timed"]
+                expect = 0 if period == 0 else (n + period - 1) // period
+                assert timed == expect, (period, side, n, timed)
+                assert (st[f"{side}_kernel_ms"] > 0) == (period > 0)
+        with pytest.raises(irlba.HipError, match="non-negative"):
+            ctx.set_profiling(-1)
+    finally:
+        ctx.set_profiling(1)
+    for period in (3, 0):
+        np.testing.assert_array_equal(runs[period][0].D, runs[1][0].D)
+        np.testing.assert_array_equal(runs[period][0].U, runs[1][0].U)
+    # pca with results left in device memory, dense input and a single component.
+    P = irlba.pca(H, True, False, 1, opt, device=True)
+    Q = irlba.pca(H, True, False, 1, opt)
+    assert tuple(P.scores.shape) == (300, 1) and tuple(P.rotation.shape) == (120, 1)
+    np.testing.assert_array_equal(P.variances.cpu().numpy(), Q.variances)
+    np.testing.assert_array_equal(P.rotation.cpu().numpy(), Q.rotation)
