@@ -249,6 +249,8 @@ __global__ __launch_bounds__(256) void k_cgs_update(const double* __restrict__ M
     if (threadIdx.x == 0) norm_part[blockIdx.x] = sq;
 }
 
+constexpr int kMaxWideCols = 1024;
+
 // The same update with 16-byte accesses (a lane holds two consecutive rows, U
 // pairs per thread, CB columns per batch); the conditions of k_cgs_dots2 apply.
 template <int U, int CB>
@@ -314,13 +316,63 @@ __global__ __launch_bounds__(256) void k_cgs_update2(const double* __restrict__ 
     if (threadIdx.x == 0) norm_part[blockIdx.x] = sq;
 }
 
+// The update for short vectors (at most 256 rows per block): 1024 threads, 4
+// row groups of 64 x 4 column groups, so that a lane has about n/4 loads, all
+// in flight at once, instead of n loads in dependent batches; the four column
+// partials of a row are summed in a fixed order through LDS.
+__global__ __launch_bounds__(1024) void k_cgs_update_wide(const double* __restrict__ M, i64 ld, int n,
+                                                           const double* __restrict__ t, double* __restrict__ f,
+                                                           i64 L, i64 chunk, double* __restrict__ norm_part) {
+    __shared__ double ts[kMaxWideCols];
+    __shared__ double part[4][256];
+    __shared__ double red[32];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) ts[i] = t[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int rg = w & 3, cg = w >> 2;
+    const i64 r0 = static_cast<i64>(blockIdx.x) * chunk;
+    const i64 r1 = min(L, r0 + chunk);
+    const int row_in = rg * 64 + lane;
+    const i64 r = r0 + row_in;
+    const int q0 = n * cg / 4, q1 = n * (cg + 1) / 4;
+    double acc = 0;
+    if (r < r1) {
+        const double* p = M + r + static_cast<i64>(q0) * ld;
+        int q = q0;
+        for (; q + 8 <= q1; q += 8) {
+            double m[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) m[c] = __builtin_nontemporal_load(p + c * ld);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) acc += m[c] * ts[q + c];
+            p += 8 * ld;
+        }
+        for (; q < q1; ++q) {
+            acc += __builtin_nontemporal_load(p) * ts[q];
+            p += ld;
+        }
+    }
+    part[cg][row_in] = acc;
+    __syncthreads();
+    double sq = 0, dummy = 0;
+    if (cg == 0 && r < r1) {
+        const double v = f[r] - ((part[0][row_in] + part[1][row_in]) + (part[2][row_in] + part[3][row_in]));
+        f[r] = v;
+        sq = v * v;
+    }
+    block_sum2(sq, dummy, red);
+    if (threadIdx.x == 0) norm_part[blockIdx.x] = sq;
+}
+
 void launch_cgs_update(const double* M, i64 ld, int n, const double* t, double* f, i64 L, double* norm_part,
                        hipStream_t s) {
     const int G = cgs_blocks(L);  // the norm finalisation sums exactly these partials
     const size_t lds = sizeof(double) * static_cast<size_t>(std::max(n, 1));
     if (lds > 60000) throw std::runtime_error("working subspace too large for the CGS update kernel");
     const i64 chunk = cgs_chunk(L, G);
-    if (chunk >= 512 && pairs_ok(M, f, ld, L)) {
+    if (chunk <= 256 && n <= kMaxWideCols) {
+        hipLaunchKernelGGL(k_cgs_update_wide, dim3(G), dim3(1024), 0, s, M, ld, n, t, f, L, chunk, norm_part);
+    } else if (chunk >= 512 && pairs_ok(M, f, ld, L)) {
         const i64 chunk2 = (chunk + 1) & ~static_cast<i64>(1);  // G * chunk2 >= L; trailing blocks write 0
         hipLaunchKernelGGL((k_cgs_update2<2, 8>), dim3(G), dim3(256), lds, s, M, ld, n, t, f, L, chunk2, norm_part);
     } else if (chunk >= 512)

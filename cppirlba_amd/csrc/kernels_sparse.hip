@@ -664,11 +664,25 @@ __global__ void k_spmv_reduce(const AT* __restrict__ partial, int n_tiles, i64 n
         gamma[0] = gsh[0];
         gamma[1] = gsh[1];
     }
-    for (i64 m = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; m < n_major;
-         m += static_cast<i64>(gridDim.x) * blockDim.x) {
+    // kSplit lanes per output line, each summing every kSplit-th tile, then a
+    // fixed xor tree: more loads in flight than a lane per line (the per-tile
+    // partials of a few tens of thousands of lines are a latency-bound read).
+    // The loop bound is uniform across a wavefront (its first line), so every
+    // lane reaches the shuffles.
+    constexpr int kSplit = 4;
+    const int sub = threadIdx.x & (kSplit - 1);
+    const i64 g = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const i64 stride = static_cast<i64>(gridDim.x) * blockDim.x / kSplit;
+    const i64 wave_first = (g & ~static_cast<i64>(kWave - 1)) / kSplit;
+    for (i64 off = 0; wave_first + off < n_major; off += stride) {
+        const i64 m = g / kSplit + off;
+        const bool live = m < n_major;
         double acc = 0;
-        for (int t = 0; t < n_tiles; ++t) acc += static_cast<double>(partial[static_cast<i64>(t) * n_major + m]);
-        epi.out[m] = apply_epilogue(epi, m, acc, gamma);
+        if (live)
+            for (int t = sub; t < n_tiles; t += kSplit) acc += static_cast<double>(partial[static_cast<i64>(t) * n_major + m]);
+        acc += __shfl_xor(acc, 1, kWave);
+        acc += __shfl_xor(acc, 2, kWave);
+        if (live && sub == 0) epi.out[m] = apply_epilogue(epi, m, acc, gamma);
     }
 }
 
@@ -702,7 +716,7 @@ static void spmv_launch_one(const TiledLines& tl, const Prologue& pro, const Epi
     IRLBA_HIP_LAUNCH_CHECK();
     if (ev1) IRLBA_HIP_CHECK(hipEventRecord(ev1, s));
     if (tl.n_tiles > 1) {
-        const i64 grid = std::max<i64>(1, std::min<i64>((tl.n_major + 255) / 256, 8192));
+        const i64 grid = std::max<i64>(1, std::min<i64>((tl.n_major * 4 + 255) / 256, 8192));  // 4 lanes per line
         hipLaunchKernelGGL(k_spmv_reduce<AT>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s,
                            static_cast<const AT*>(partial), tl.n_tiles, tl.n_major, gamma_part, epi);
         IRLBA_HIP_LAUNCH_CHECK();
