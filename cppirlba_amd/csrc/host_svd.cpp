@@ -123,20 +123,38 @@ void jacobi_svd_base(const HostMat& A, HostMat& U, std::vector<double>& S, HostM
 
 }  // namespace
 
-void host_svd(const HostMat& A, HostMat& U, std::vector<double>& S, HostMat& V) {
-    if (A.nr < A.nc) {
-        HostMat T(A.nc, A.nr);
-        for (i64 c = 0; c < A.nc; ++c)
-            for (i64 r = 0; r < A.nr; ++r) T(c, r) = A(r, c);
-        host_svd(T, V, S, U);
-        return;
-    }
+namespace {
+
+bool cpu_has_avx2_fma() {
     static const bool fast = [] {
         __builtin_cpu_init();
         return __builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma");
     }();
-    if (fast) jacobi_svd_avx2(A, U, S, V);
+    return fast;
+}
+
+void run_variant(const HostMat& A, HostMat& U, std::vector<double>& S, HostMat& V, bool avx2) {
+    if (A.nr < A.nc) {
+        HostMat T(A.nc, A.nr);
+        for (i64 c = 0; c < A.nc; ++c)
+            for (i64 r = 0; r < A.nr; ++r) T(c, r) = A(r, c);
+        run_variant(T, V, S, U, avx2);
+        return;
+    }
+    if (avx2) jacobi_svd_avx2(A, U, S, V);
     else jacobi_svd_base(A, U, S, V);
+}
+
+}  // namespace
+
+void host_svd(const HostMat& A, HostMat& U, std::vector<double>& S, HostMat& V) {
+    run_variant(A, U, S, V, cpu_has_avx2_fma());
+}
+
+bool host_svd_variant(const HostMat& A, HostMat& U, std::vector<double>& S, HostMat& V, int variant) {
+    if (variant == 1 && !cpu_has_avx2_fma()) return false;
+    run_variant(A, U, S, V, variant == 1);
+    return true;
 }
 
 }  // namespace irlba_hip

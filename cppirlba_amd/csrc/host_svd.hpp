@@ -25,4 +25,9 @@ struct HostMat {
 // A = U diag(S) V'.  Internal to the library.
 __attribute__((visibility("hidden"))) void host_svd(const HostMat& A, HostMat& U, std::vector<double>& S, HostMat& V);
 
+// The same with a chosen build: 0 the baseline x86-64 code, 1 the AVX2 + FMA
+// code (only where the CPU has them; returns false otherwise).  For tests.
+__attribute__((visibility("hidden"))) bool host_svd_variant(const HostMat& A, HostMat& U, std::vector<double>& S,
+                                                            HostMat& V, int variant);
+
 }  // namespace irlba_hip
