@@ -135,6 +135,15 @@ struct irlba_hip_context {
     // buffer serves them all.
     std::shared_ptr<void> ws;
     size_t ws_bytes = 0;
+    // A second stream (default priority) for host-to-device copies that should
+    // overlap the compute stream, and an event to order them; created on first
+    // use (a stream costs milliseconds to create), destroyed with the context.
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_done = nullptr;
+    void ensure_copy_stream() {
+        if (!copy_stream) IRLBA_HIP_CHECK(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+        if (!copy_done) IRLBA_HIP_CHECK(hipEventCreateWithFlags(&copy_done, hipEventDisableTiming));
+    }
     // A device int cleared in stream order, for the error bits of checking kernels.
     std::shared_ptr<int> flag_buf;
     int* flag() {
@@ -830,6 +839,7 @@ struct Engine {
         IRLBA_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&yt_host), sizeof(double) * 2 * work * ldy));
         IRLBA_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&norms_host), sizeof(double) * (2 * work + 4)));
         IRLBA_HIP_CHECK(hipEventCreateWithFlags(&norms_ready, hipEventDisableTiming));
+        ctx->ensure_copy_stream();
         IRLBA_HIP_CHECK(hipMemsetAsync(W.get(), 0, sizeof(double) * nr * work, s));
         IRLBA_HIP_CHECK(hipMemsetAsync(V.get(), 0, sizeof(double) * nc * work, s));
         IRLBA_HIP_CHECK(hipMemsetAsync(dnorm.get(), 0, sizeof(double) * (2 * work + 4), s));
@@ -979,9 +989,16 @@ struct Engine {
         for (i64 r = 0; r < work; ++r)
             for (i64 c = 0; c < k; ++c) yt[r * ldy + c] = Y(r, c);
     }
+    // The copy runs on a stream of its own, so that it overlaps the product
+    // queued before the host SVD instead of waiting behind it on the copy
+    // engine; the compute stream waits for it before the rotations.  (Every
+    // earlier reader of the device slots precedes the norms read-back the host
+    // has waited for.)
     void upload_y(int nslots) {
         IRLBA_HIP_CHECK(hipMemcpyAsync(Yt.get(), yt_host, sizeof(double) * nslots * work * ldy, hipMemcpyHostToDevice,
-                                       s));
+                                       ctx->copy_stream));
+        IRLBA_HIP_CHECK(hipEventRecord(ctx->copy_done, ctx->copy_stream));
+        IRLBA_HIP_CHECK(hipStreamWaitEvent(s, ctx->copy_done, 0));
     }
     // out[:, :k] = M[:, :work] Y[:, :k] with the staged slot's Y (out == M allowed).
     void rotate_staged(int slot, double* M, i64 L, i64 k, double* out, i64 ld_out) {
@@ -994,6 +1011,7 @@ struct Engine {
     hipEvent_t norms_ready = nullptr;
     ~Engine() {
         (void)hipStreamSynchronize(s);
+        (void)hipStreamSynchronize(ctx->copy_stream);
         if (yt_host) (void)hipHostFree(yt_host);
         if (norms_host) (void)hipHostFree(norms_host);
         if (norms_ready) (void)hipEventDestroy(norms_ready);
@@ -1495,6 +1513,11 @@ int irlba_hip_context_destroy(irlba_hip_context* ctx) {
             (void)hipEventDestroy(p.b);
         }
         for (auto e : ctx->pool) (void)hipEventDestroy(e);
+        if (ctx->copy_stream) {
+            (void)hipStreamSynchronize(ctx->copy_stream);
+            (void)hipStreamDestroy(ctx->copy_stream);
+        }
+        if (ctx->copy_done) (void)hipEventDestroy(ctx->copy_done);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
     });
