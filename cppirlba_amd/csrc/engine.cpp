@@ -1525,17 +1525,22 @@ int irlba_hip_context_destroy(irlba_hip_context* ctx) {
 
 int irlba_hip_context_synchronize(irlba_hip_context* ctx) {
     return guarded([&] {
+        require(ctx != nullptr, "null context");
         DeviceGuard g(ctx->device);
         ctx->sync();
     });
 }
 
 int irlba_hip_get_stats(const irlba_hip_context* ctx, irlba_hip_stats* out) {
-    return guarded([&] { *out = ctx->stats; });
+    return guarded([&] {
+        require(ctx && out, "null argument");
+        *out = ctx->stats;
+    });
 }
 
 int irlba_hip_context_set_profiling(irlba_hip_context* ctx, int period) {
     return guarded([&] {
+        require(ctx != nullptr, "null context");
         if (period < 0) throw std::runtime_error("profiling period must be non-negative");
         ctx->profiling = period;
         ctx->profile_seq[0] = ctx->profile_seq[1] = 0;

@@ -71,6 +71,28 @@ def test_no_device_reports_error_not_crash():
         irlba.Context(0)
 
 
+def test_null_handles_report_errors():
+    """Entry points given a null context or matrix return an error status with a
+    message instead of dereferencing it; the destroy calls accept null."""
+    from cppirlba_amd._lib import Stats, lib
+
+    L = lib()
+    i64 = ctypes.c_int64
+    r, c = i64(), i64()
+    st = Stats()
+    calls = {
+        "context_synchronize": lambda: L.irlba_hip_context_synchronize(None),
+        "get_stats": lambda: L.irlba_hip_get_stats(None, ctypes.byref(st)),
+        "set_profiling": lambda: L.irlba_hip_context_set_profiling(None, 1),
+        "matrix_dims": lambda: L.irlba_hip_matrix_dims(None, ctypes.byref(r), ctypes.byref(c)),
+    }
+    for name, call in calls.items():
+        assert call() != 0, name
+        assert L.irlba_hip_last_error().decode().startswith("null"), name
+    assert L.irlba_hip_context_destroy(None) == 0
+    assert L.irlba_hip_matrix_destroy(None) == 0
+
+
 def test_torch_binding_registers_operators():
     """libirlba_torch.so (the torch::autograd::Function binding) loads and
     registers its operators; without a GPU a call raises instead of falling back."""
