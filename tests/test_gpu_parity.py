@@ -85,6 +85,90 @@ def test_sparse_mixed_precision_products():
     assert _rel(H32.adjoint_multiply(u), A.multiply(u, adjoint=True)) < 2e-6
 
 
+@pytest.mark.parametrize("column_major", [True, False])
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_sparse_without_entries(column_major, storage):
+    # A 50 x 40 matrix with no stored entries: zero products and realize,
+    # column statistics (0, 1) (pca.hpp:85-109 with the zero-variance rule),
+    # zeros from the exact path, and the reference's null-space error from the
+    # Lanczos start (lanczos.hpp:83) in compute() and pca(), as the oracle raises.
+    nr, nc = 50, 40
+    ptr = np.zeros((nc if column_major else nr) + 1, dtype=np.int64)
+    idx = np.zeros(0, dtype=np.int32)
+    vals = np.zeros(0, dtype=np.float32 if storage == "f32" else np.float64)
+    H = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, column_major=column_major, storage=storage)
+    A = O.sparse(nr, nc, np.zeros(0), idx, ptr, column_major, 1)
+    assert not np.any(H.multiply(np.ones(nc))) and not np.any(H.adjoint_multiply(np.ones(nr)))
+    assert not np.any(H.realize())
+    c, s = irlba.column_stats(H)
+    assert not np.any(c) and np.all(s == 1)
+    init = O.create_random_vector(nc, 3)
+    with pytest.raises(RuntimeError, match="null space"):
+        O.compute(A, 3, initial=init)
+    with pytest.raises(irlba.HipError, match="starting vector near the null space"):
+        irlba.compute(H, 3, irlba.Options(initial=init))
+    with pytest.raises(irlba.HipError, match="starting vector near the null space"):
+        irlba.pca(H, True, True, 2, irlba.Options(initial=init))
+    r = irlba.compute(H, 25)
+    assert r.converged and r.iterations == 0 and r.D.shape == (25,) and not np.any(r.D)
+
+
+def _ragged_matrix():
+    # 300 x 200 with 287 empty rows and 157 empty columns: a half-filled 10 x 40
+    # block plus three isolated entries (first row/column, last row/column).
+    rng = np.random.default_rng(7)
+    M = np.zeros((300, 200))
+    M[5:15, 150:190] = (rng.random((10, 40)) < 0.5) * rng.standard_normal((10, 40))
+    M[250, 3], M[299, 199], M[0, 0] = 2.0, -1.5, 0.5
+    return M
+
+
+def _compressed(M, column_major):
+    lines = M if not column_major else M.T
+    nz = [np.nonzero(line)[0] for line in lines]
+    ptr = np.zeros(len(nz) + 1, dtype=np.int64)
+    ptr[1:] = np.cumsum([len(z) for z in nz])
+    idx = np.concatenate(nz).astype(np.int32)
+    vals = np.concatenate([line[z] for line, z in zip(lines, nz)])
+    return vals, idx, ptr
+
+
+@pytest.mark.parametrize("column_major", [True, False])
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_sparse_ragged_vs_oracle(column_major, storage):
+    # Empty rows and columns everywhere: products against the dense GEMV, and
+    # compute() / pca() against the oracle from the same start vector.  FP32
+    # storage is compared with the oracle on the FP32-rounded values.
+    M = _ragged_matrix()
+    if storage == "f32":
+        M = M.astype(np.float32).astype(np.float64)
+    vals, idx, ptr = _compressed(M, column_major)
+    H = irlba.HipSparseMatrix(300, 200, vals.astype(np.float32) if storage == "f32" else vals, idx, ptr,
+                              column_major=column_major, storage=storage)
+    A = O.sparse(300, 200, vals, idx, ptr, column_major, 1)
+    tp, ts = (1e-13, 1e-8) if storage == "f64" else (2e-6, 1e-5)
+    v = O.create_random_vector(200, 11)
+    u = O.create_random_vector(300, 12)
+    assert _rel(H.multiply(v), M @ v) < tp
+    assert _rel(H.adjoint_multiply(u), M.T @ u) < tp
+    np.testing.assert_array_equal(H.realize(), M)
+    init = O.create_random_vector(200, 5)
+    res = irlba.compute(H, 4, irlba.Options(initial=init))
+    oU, oV, oD, oconv, oit = O.compute(A, 4, initial=init)
+    assert res.converged and oconv
+    assert_equal_vectors(res.D, oD, ts)
+    assert_equal_vectors(res.D, np.linalg.svd(M, compute_uv=False)[:4], ts)
+    # Entries of the singular vectors on rows orthogonal to the top four sit at
+    # rounding level (1e-17 .. 1e-13), so vectors compare with an absolute bound.
+    tv = 1e-9 if storage == "f64" else 1e-5
+    np.testing.assert_allclose(align_signs(res.U, oU), oU, rtol=0, atol=tv)
+    np.testing.assert_allclose(align_signs(res.V, oV), oV, rtol=0, atol=tv)
+    pres = irlba.pca(H, True, True, 3, irlba.Options(initial=init))
+    oS, oR, ovar, pconv, _ = O.pca(A, True, True, 3, initial=init)
+    assert pres.converged and pconv
+    assert_equal_vectors(pres.variances, ovar, ts)
+
+
 def test_sparse_rejects_bad_input():
     vals = np.ones(3)
     with pytest.raises(irlba.HipError, match="indices"):
