@@ -188,6 +188,33 @@ def test_generator_bit_identical_to_oracle():
     np.testing.assert_array_equal(v, ov)
 
 
+def test_products_beyond_2e31_entries():
+    # 30000 x 1,000,000 at 8%: ~2.4e9 stored entries, past the int32 range, so
+    # every slot offset of the last columns (and of the last tiles of the row
+    # layout) lies above 2^31.  FP32 values with FP64 accumulation, against the
+    # oracle on the first and the last 2000 columns (the same generator on the
+    # CPU, bit-identical), and the adjoint identity u'(A v) = (A'u)'v over all of it.
+    nr, nc, d, seed, w = 30000, 1000000, 0.08, 3, 2000
+    H = irlba.generate_sparse(nr, nc, d, seed, storage="f32", accumulate="f64")
+    lay = H.layout(False)
+    assert lay["nnz"] > 2 ** 31 and lay["slots"] > 2 ** 31
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(nr)
+    z = H.adjoint_multiply(u)
+    for c0 in (0, nc - w):
+        vals, idx, ptr = O.gen_nb_sparse(nr, nc, d, seed, line0=c0, line1=c0 + w, f32=True)
+        A = O.sparse(nr, w, vals, idx, ptr, True, 1)
+        vt = rng.standard_normal(w)
+        v = np.zeros(nc)
+        v[c0:c0 + w] = vt
+        assert _rel(H.multiply(v), A.multiply(vt)) < 1e-12
+        assert _rel(z[c0:c0 + w], A.multiply(u, adjoint=True)) < 1e-12
+    v = rng.standard_normal(nc)
+    a, b = u @ H.multiply(v), z @ v
+    assert abs(a - b) <= 1e-10 * (abs(a) + abs(b))
+    del H
+
+
 # ---------------------------------------------------------------- dense products
 @pytest.mark.parametrize("nr,nc", [(20, 10), (1000, 200), (33, 1001), (4099, 17), (40000, 300)])
 @pytest.mark.parametrize("order", ["F", "C"])
