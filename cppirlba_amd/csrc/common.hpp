@@ -247,6 +247,8 @@ void launch_dense_matvec(const DenseDesc& d, bool adjoint, const Prologue& pro, 
                          hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_dense_realize(const DenseDesc& d, double* dense, hipStream_t s);
 void launch_dense_colstats(const DenseDesc& d, double* center_out, double* scale_out, hipStream_t s);
+// Dense array conversion between FP64, FP32 and BF16 (round to nearest even).
+void launch_convert_values(const void* in, StorageType in_t, void* out, StorageType out_t, i64 n, hipStream_t s);
 i64 dense_partial_elems(const DenseDesc& d);
 
 // Lanczos vector kernels (FP64).

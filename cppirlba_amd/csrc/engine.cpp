@@ -282,6 +282,8 @@ namespace {
 
 std::shared_ptr<double> upload_vec(irlba_hip_context* ctx, const double* src, i64 n, bool device_ptrs) {
     auto d = dalloc<double>(n);
+    // The caller's device vector may still be written by work on another stream.
+    if (device_ptrs) IRLBA_HIP_CHECK(hipDeviceSynchronize());
     IRLBA_HIP_CHECK(hipMemcpyAsync(d.get(), src, sizeof(double) * n,
                                    device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
     IRLBA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -588,6 +590,8 @@ std::shared_ptr<SparseBase> make_sparse(irlba_hip_context* ctx, i64 nrow, i64 nc
     hipStream_t s = ctx->stream;
     const i64 primary = column_major ? ncol : nrow, secondary = column_major ? nrow : ncol;
     i64 nnz = 0;
+    // The caller's device arrays may still be written by work on another stream.
+    if (device_ptrs) IRLBA_HIP_CHECK(hipDeviceSynchronize());
     if (device_ptrs) IRLBA_HIP_CHECK(hipMemcpy(&nnz, pointers + primary, sizeof(i64), hipMemcpyDeviceToHost));
     else nnz = pointers[primary];
     require(nnz >= 0, "sparse pointers must end at a non-negative count");
@@ -1577,34 +1581,27 @@ int irlba_hip_dense_create(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, i
         d->nc = ncol;
         const i64 n = nrow * ncol;
         d->data = dalloc_bytes(storage_bytes(st) * n);
+        // A caller's device buffer may still be written by work queued on
+        // another stream (the engine's stream does not wait for them).
+        if (device_pointers) IRLBA_HIP_CHECK(hipDeviceSynchronize());
         if (in_t == st) {
-            IRLBA_HIP_CHECK(hipMemcpy(d->data.get(), values, storage_bytes(st) * n,
-                                      device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+            // On the engine's stream and waited for: a device-to-device hipMemcpy
+            // returns before the copy ends, and the products run on a stream that
+            // does not wait for the null stream (a 5 GB copy was still running
+            // when the first product read the matrix).
+            IRLBA_HIP_CHECK(hipMemcpyAsync(d->data.get(), values, storage_bytes(st) * n,
+                                           device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                           ctx->stream));
+            IRLBA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        } else if (device_pointers) {
+            launch_convert_values(values, in_t, d->data.get(), st, n, ctx->stream);
+            IRLBA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         } else {
-            // Convert on the device through the sparse scatter's value conversion: stage as given.
+            // Host input of another type: copied as given, converted on the device.
             auto raw = dalloc_bytes(storage_bytes(in_t) * n);
-            IRLBA_HIP_CHECK(hipMemcpy(raw.get(), values, storage_bytes(in_t) * n,
-                                      device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
-            std::vector<char> tmp(storage_bytes(in_t) * n);
-            IRLBA_HIP_CHECK(hipMemcpy(tmp.data(), raw.get(), tmp.size(), hipMemcpyDeviceToHost));
-            std::vector<char> conv(storage_bytes(st) * n);
-            for (i64 i = 0; i < n; ++i) {
-                const double v = in_t == ST_F64 ? reinterpret_cast<const double*>(tmp.data())[i]
-                                                : static_cast<double>(reinterpret_cast<const float*>(tmp.data())[i]);
-                if (st == ST_F64) reinterpret_cast<double*>(conv.data())[i] = v;
-                else if (st == ST_F32) reinterpret_cast<float*>(conv.data())[i] = static_cast<float>(v);
-                else {
-                    // round-to-nearest-even FP32 -> BF16
-                    float f = static_cast<float>(v);
-                    std::uint32_t u;
-                    std::memcpy(&u, &f, 4);
-                    std::uint16_t b;
-                    if ((u & 0x7fffffffu) > 0x7f800000u) b = static_cast<std::uint16_t>((u >> 16) | 0x40u);
-                    else b = static_cast<std::uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
-                    reinterpret_cast<std::uint16_t*>(conv.data())[i] = b;
-                }
-            }
-            IRLBA_HIP_CHECK(hipMemcpy(d->data.get(), conv.data(), conv.size(), hipMemcpyHostToDevice));
+            IRLBA_HIP_CHECK(hipMemcpy(raw.get(), values, storage_bytes(in_t) * n, hipMemcpyHostToDevice));
+            launch_convert_values(raw.get(), in_t, d->data.get(), st, n, ctx->stream);
+            IRLBA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         }
         d->d.nr = nrow;
         d->d.nc = ncol;

@@ -374,6 +374,39 @@ void launch_dense_colstats(const DenseDesc& d, double* center_out, double* scale
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
+// ------------------------------------------------------------------ storage conversion
+// Element-wise conversion of a dense array between FP64, FP32 and BF16 on the
+// device.  FP64 -> BF16 goes through FP32 (both steps round to nearest even);
+// the BF16 rounding adds 0x7fff plus the lowest kept bit to the FP32 pattern
+// and keeps the top half, and a NaN keeps its sign and top payload bits with
+// the quiet bit set.
+__device__ __forceinline__ std::uint16_t bf16_bits_rne(float f) {
+    const std::uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<std::uint16_t>((u >> 16) | 0x40u);
+    return static_cast<std::uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__global__ void k_convert_values(const void* __restrict__ in, int in_t, void* __restrict__ out, int out_t, i64 n) {
+    for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<i64>(gridDim.x) * blockDim.x) {
+        double v;
+        if (in_t == ST_F64) v = static_cast<const double*>(in)[i];
+        else if (in_t == ST_F32) v = static_cast<const float*>(in)[i];
+        else v = __uint_as_float(static_cast<std::uint32_t>(static_cast<const std::uint16_t*>(in)[i]) << 16);
+        if (out_t == ST_F64) static_cast<double*>(out)[i] = v;
+        else if (out_t == ST_F32) static_cast<float*>(out)[i] = static_cast<float>(v);
+        else static_cast<std::uint16_t*>(out)[i] = bf16_bits_rne(static_cast<float>(v));
+    }
+}
+
+void launch_convert_values(const void* in, StorageType in_t, void* out, StorageType out_t, i64 n, hipStream_t s) {
+    if (n <= 0) return;
+    const i64 grid = std::max<i64>(1, std::min<i64>((n + 255) / 256, 65536));
+    hipLaunchKernelGGL(k_convert_values, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, in, static_cast<int>(in_t),
+                       out, static_cast<int>(out_t), n);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
 i64 dense_partial_elems(const DenseDesc& d) {
     // Upper bound on the partial buffer used by either direction.
     const i64 n_lines = d.col_major ? d.nc : d.nr;

@@ -8,7 +8,11 @@
  * (include/irlba/compute.hpp:156,159,174,184; lanczos.hpp:83; pca.hpp:72,80).
  *
  * Arrays are column-major.  `device_pointers` = 0 means host memory (copied
- * in/out by the call); 1 means device memory on the context's GPU.
+ * in/out by the call); 1 means device memory on the context's GPU.  The
+ * constructors (sparse, dense, centered, scaled) synchronise the device before
+ * they read a caller's device arrays; the other calls run on the context's
+ * stream, which does not wait for other streams, so their device inputs must
+ * be complete when the call is made (e.g. after torch.cuda.synchronize()).
  * One context = one device + one stream; calls on a context are stream
  * ordered and a context must not be used from two threads at once (the
  * single-caller rule of a reference Workspace, include/irlba/Matrix/scaled.hpp:49).

@@ -236,6 +236,66 @@ def test_dense_products(nr, nc, order):
     assert _rel(Hb.adjoint_multiply(u), A.T @ u) < 2e-2
 
 
+def test_dense_storage_conversions_bit_exact():
+    # Every input type to every storage type, host and device input: the stored
+    # matrix (read back by realize) equals the round-to-nearest-even conversion
+    # torch makes (FP64 -> BF16 through FP32), bit for bit.
+    import torch
+
+    rng = np.random.default_rng(21)
+    A64 = rng.standard_normal((37, 23)) * np.exp(rng.uniform(-20, 20, (37, 23)))
+    ref = {"f64": torch.from_numpy(A64)}
+    ref["f32"] = ref["f64"].float()
+    ref["bf16"] = ref["f32"].to(torch.bfloat16)
+    for src in ("f64", "f32", "bf16"):
+        for dst in ("f64", "f32", "bf16"):
+            expect = ref[src].to({"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[dst])
+            expect = expect.double().numpy()
+            inputs = [ref[src].cuda(), ref[src].cuda().t().contiguous().t()]
+            if src != "bf16":
+                inputs.append(ref[src].numpy())
+            for x in inputs:
+                H = irlba.HipDenseMatrix(x, storage=dst)
+                np.testing.assert_array_equal(H.realize(), expect, err_msg=f"{src} -> {dst}")
+    # and a product on converted storage
+    H = irlba.HipDenseMatrix(ref["bf16"].cuda(), storage="f32")
+    v = rng.standard_normal(23)
+    assert _rel(H.multiply(v), ref["bf16"].double().numpy() @ v) < 2e-6
+
+
+@pytest.mark.parametrize("order", ["C", "F"])
+def test_dense_products_beyond_2e31_elements(order):
+    # 200000 x 11000 FP32 (2.2e9 elements, past the int32 range) made on the
+    # device with torch and handed over as a device pointer, row- or
+    # column-major.  Both products against FP64 torch products on 20000-row slices.
+    import torch
+
+    nr, nc = 200000, 11000
+    g = torch.Generator(device="cuda").manual_seed(17)
+    T = torch.randn(nr, nc, dtype=torch.float32, device="cuda", generator=g)
+    if order == "F":
+        T = T.t().contiguous().t()  # column-major storage, same values
+    H = irlba.HipDenseMatrix(T, storage="f32")
+    rng = np.random.default_rng(8)
+    v = rng.standard_normal(nc)
+    u = rng.standard_normal(nr)
+    y = H.multiply(v)
+    z = H.adjoint_multiply(u)
+    vd = torch.from_numpy(v).cuda()
+    ud = torch.from_numpy(u).cuda()
+    y_ref = torch.empty(nr, dtype=torch.float64, device="cuda")
+    z_ref = torch.zeros(nc, dtype=torch.float64, device="cuda")
+    for r0 in range(0, nr, 20000):
+        blk = T[r0:r0 + 20000].double()
+        y_ref[r0:r0 + 20000] = blk @ vd
+        z_ref += blk.t() @ ud[r0:r0 + 20000]
+        del blk
+    assert _rel(y, y_ref.cpu().numpy()) < 2e-6
+    assert _rel(z, z_ref.cpu().numpy()) < 2e-6
+    del H, T
+    torch.cuda.empty_cache()
+
+
 # ---------------------------------------------------------------- decorators
 @pytest.mark.parametrize("column", [True, False])
 @pytest.mark.parametrize("divide", [True, False])
