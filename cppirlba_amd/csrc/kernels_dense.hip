@@ -229,7 +229,15 @@ void dense_matvec_typed(const DenseDesc& d, bool adjoint, const Prologue& pro, c
 
     constexpr int NV = Vec16<VT>::N;
     int splits = 1;
-    if (dot) {
+    if (n_lines == 0 || Ln == 0) {
+        // No entries: the product is zero, and only the epilogue (centring,
+        // scaling, Lanczos axpy) acts on it.  A timed product still records
+        // its two events.
+        if (ev0) IRLBA_HIP_CHECK(hipEventRecord(ev0, s));
+        if (ev1) IRLBA_HIP_CHECK(hipEventRecord(ev1, s));
+        if (n_out == 0) return;
+        splits = 0;
+    } else if (dot) {
         constexpr int R = 4;
         const i64 groups = (n_lines + R - 1) / R;
         // Segment the lines so that there are enough blocks to fill the chip.
@@ -329,7 +337,6 @@ __global__ void k_dense_colstats(const VT* __restrict__ A, i64 nr, i64 nc, int c
 void launch_dense_matvec(const DenseDesc& d, bool adjoint, const Prologue& pro, const Epilogue& epi, void* xs_scratch,
                          double* gamma_scratch, void* partial, i64 partial_cap, hipStream_t s, hipEvent_t ev0,
                          hipEvent_t ev1) {
-    if (d.nr == 0 || d.nc == 0) return;
     if (d.st == ST_F64)
         dense_matvec_typed<double, double>(d, adjoint, pro, epi, xs_scratch, gamma_scratch, partial, partial_cap, s,
                                            ev0, ev1);

@@ -232,6 +232,11 @@ void dense_block_typed(const DenseDesc& d, bool adjoint, const void* Xv, double*
     const i64 n_lines = d.col_major ? d.nc : d.nr;
     const i64 Ln = d.col_major ? d.nr : d.nc;
     const bool dot = d.col_major ? adjoint : !adjoint;
+    if (n_lines == 0 || Ln == 0) {  // no entries: a zero product
+        const i64 n_out = dot ? n_lines : Ln;
+        if (n_out > 0) IRLBA_HIP_CHECK(hipMemsetAsync(Y, 0, sizeof(double) * n_out * KP, s));
+        return;
+    }
     if (dot) {
         const i64 grid = (n_lines + 1) / 2;
         hipLaunchKernelGGL((k_dense_block_dot<VT, FT, KP>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, A,
@@ -271,7 +276,7 @@ void dense_block_kp(const DenseDesc& d, bool adjoint, int kp, const void* X, dou
 i64 dense_block_scratch(const DenseDesc& d, int kp) {
     const i64 n_lines = d.col_major ? d.nc : d.nr;
     const i64 Ln = d.col_major ? d.nr : d.nc;
-    const i64 xb = (Ln + 255) / 256;
+    const i64 xb = std::max<i64>(1, (Ln + 255) / 256);  // at least 1: a matrix may have no lines or no entries per line
     const i64 chunks = std::max<i64>(1, std::min<i64>(std::max<i64>(1, 1024 / xb), (n_lines + 31) / 32));
     return Ln * kp * chunks;  // the axpy orientation's partials (the dot one needs none)
 }

@@ -236,6 +236,84 @@ def test_dense_products(nr, nc, order):
     assert _rel(Hb.adjoint_multiply(u), A.T @ u) < 2e-2
 
 
+@pytest.mark.parametrize("kind", ["sparse", "sparse-multitile", "sparse-f32", "dense-C", "dense-F", "decorated"])
+def test_products_overwrite_their_output(kind):
+    # Workspace::multiply fully overwrites a pre-sized output (interface.hpp:49,87;
+    # re-zeroing checked in sparse.cpp:119-124, centered.cpp:56-58): device
+    # outputs filled with NaN, then products twice in a row, against the
+    # products into fresh host buffers.
+    import ctypes
+
+    import torch
+    from cppirlba_amd._lib import check, lib
+
+    if kind.startswith("sparse") or kind == "decorated":
+        nr, nc = (64, 40000) if kind == "sparse-multitile" else (300, 200)
+        vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.05, 4, f32=kind == "sparse-f32")
+        H = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr)
+        if kind == "decorated":
+            rng = np.random.default_rng(2)
+            H = irlba.ScaledMatrix(irlba.CenteredMatrix(H, rng.standard_normal(nc)), rng.uniform(0.5, 2, nc))
+    else:
+        nr, nc = 300, 200
+        A = np.asarray(np.random.default_rng(3).standard_normal((nr, nc)), order="F" if kind == "dense-F" else "C")
+        H = irlba.HipDenseMatrix(A)
+    for adjoint, (n_in, n_out) in ((0, (nc, nr)), (1, (nr, nc))):
+        xh = np.random.default_rng(9 + adjoint).standard_normal(n_in)
+        ref = H.multiply(xh, adjoint=bool(adjoint))
+        x = torch.from_numpy(xh).cuda()
+        out = torch.full((n_out,), float("nan"), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        for _ in range(2):
+            check(lib().irlba_hip_multiply(H.handle, adjoint, ctypes.c_void_p(x.data_ptr()),
+                                           ctypes.c_void_p(out.data_ptr()), 1))
+            np.testing.assert_array_equal(out.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("shape", [(5, 0), (0, 5)])
+@pytest.mark.parametrize("kind", ["dense-C", "dense-F", "dense-f32", "sparse-csc", "sparse-csr"])
+def test_empty_dimension_products(shape, kind):
+    # A matrix with no rows or no columns: products are zero vectors of the
+    # output length (Eigen's result for the reference's SimpleMatrix), block
+    # products likewise, and compute() raises the reference's error, as the
+    # oracle does.
+    nr, nc = shape
+    if kind.startswith("dense"):
+        A = np.zeros(shape, order="F" if kind == "dense-F" else "C")
+        H = irlba.HipDenseMatrix(A.astype(np.float32) if kind == "dense-f32" else A,
+                                 storage="f32" if kind == "dense-f32" else "f64")
+    else:
+        cm = kind == "sparse-csc"
+        ptr = np.zeros((nc if cm else nr) + 1, dtype=np.int64)
+        H = irlba.HipSparseMatrix(nr, nc, np.zeros(0), np.zeros(0, dtype=np.int32), ptr, column_major=cm)
+    y = H.multiply(np.ones(nc))
+    z = H.adjoint_multiply(np.ones(nr))
+    assert y.shape == (nr,) and z.shape == (nc,) and not y.any() and not z.any()
+    # The output is overwritten whatever it held (interface.hpp:49,87; the
+    # reference's tests fill it with garbage first, sparse.cpp:121-125).
+    import ctypes
+
+    import torch
+    from cppirlba_amd._lib import check, lib
+
+    for adjoint, (n_in, n_out) in ((0, (nc, nr)), (1, (nr, nc))):
+        x = torch.ones(max(n_in, 1), dtype=torch.float64, device="cuda")
+        out = torch.full((max(n_out, 1),), float("nan"), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        check(lib().irlba_hip_multiply(H.handle, adjoint, ctypes.c_void_p(x.data_ptr()),
+                                       ctypes.c_void_p(out.data_ptr()), 1))
+        assert not out[:n_out].cpu().numpy().any()
+    Yb = H.multiply_block(np.ones((nc, 3)))
+    Zb = H.multiply_block(np.ones((nr, 3)), adjoint=True)
+    assert Yb.shape == (nr, 3) and Zb.shape == (nc, 3) and not Yb.any() and not Zb.any()
+    assert H.realize().shape == shape
+    msg = "cannot be greater than the smaller matrix dimension"
+    with pytest.raises(RuntimeError, match=msg):
+        O.compute(O.dense(np.zeros(shape)), 1)
+    with pytest.raises(irlba.HipError, match=msg):
+        irlba.compute(H, 1)
+
+
 def test_dense_storage_conversions_bit_exact():
     # Every input type to every storage type, host and device input: the stored
     # matrix (read back by realize) equals the round-to-nearest-even conversion
