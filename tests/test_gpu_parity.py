@@ -179,6 +179,29 @@ def test_sparse_rejects_bad_input():
         irlba.HipSparseMatrix(4, 2, vals, np.array([1, 1, 2]), np.array([0, 2, 3]))
 
 
+@pytest.mark.parametrize("column_major", [True, False])
+@pytest.mark.parametrize("f32", [False, True])
+def test_unsorted_lines_match_sorted(column_major, f32):
+    # The reference's products take the entries of a line in any order
+    # (sparse.hpp:185-267 only sums them); the device layout sorts each segment,
+    # so shuffled lines give bit-identical products to sorted ones.
+    n_minor, n_major = 3000, 2000
+    vals, idx, ptr = O.gen_nb_sparse(n_minor, n_major, 0.05, 12, f32=f32)
+    rng = np.random.default_rng(4)
+    perm = np.arange(idx.size)
+    for c in range(n_major):
+        rng.shuffle(perm[ptr[c]:ptr[c + 1]])
+    nr, nc = (n_minor, n_major) if column_major else (n_major, n_minor)
+    Hs = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, column_major=column_major)
+    Hu = irlba.HipSparseMatrix(nr, nc, vals[perm], idx[perm], ptr, column_major=column_major)
+    assert Hs.layout(False)["delta"] and Hu.layout(False)["delta"]
+    v = rng.standard_normal(nc)
+    u = rng.standard_normal(nr)
+    np.testing.assert_array_equal(Hu.multiply(v), Hs.multiply(v))
+    np.testing.assert_array_equal(Hu.adjoint_multiply(u), Hs.adjoint_multiply(u))
+    np.testing.assert_array_equal(Hu.realize(), Hs.realize())
+
+
 def test_generator_bit_identical_to_oracle():
     nr, nc, d, seed = 700, 300, 0.04, 99
     v, i, p = irlba.generate_csc_host(nr, nc, d, seed, "f32")
