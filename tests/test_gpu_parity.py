@@ -691,6 +691,42 @@ def test_callback_matrix_matches_device_matrix():
     assert_equal_column_vectors(r1.U, r2.U, 1e-6)
 
 
+class FailingMatrix(NumpyMatrix):
+    """Raises from its product after `ok` successful calls."""
+
+    def __init__(self, A, ok):
+        super().__init__(A)
+        self.left = ok
+
+    def multiply(self, x):
+        self.left -= 1
+        if self.left < 0:
+            raise ArithmeticError("product failed on purpose")
+        return self.A @ x
+
+
+def test_failing_callback_leaves_context_usable():
+    # An exception in a caller-defined product ends the solve with that
+    # exception, at the start and in the middle of a sweep; the same context then
+    # runs a profiled device solve whose results match a fresh run.
+    ctx = irlba.default_context()
+    A = O.create_random_matrix(120, 90)
+    init = O.create_random_vector(90, 6)
+    opt = irlba.Options(initial=init)
+    for ok in (0, 7):
+        with pytest.raises(ArithmeticError, match="on purpose"):
+            irlba.compute(FailingMatrix(A, ok), 5, opt)
+    ref = irlba.compute(NumpyMatrix(A), 5, opt)
+    before = ctx.stats()["forward_timed"]
+    ctx.set_profiling(2)
+    try:
+        res = irlba.compute(irlba.HipDenseMatrix(A), 5, opt)
+        assert ctx.stats()["forward_timed"] > before
+    finally:
+        ctx.set_profiling(1)  # the default: every product timed
+    assert_equal_vectors(res.D, ref.D, 1e-10)
+
+
 def test_sampled_product_timing_and_device_results():
     # Context.set_profiling(period): one product in `period` of each direction
     # carries HIP events; the counts and the results do not depend on it.
