@@ -280,6 +280,16 @@ struct irlba_hip_matrix {
 
 namespace {
 
+// A copy on the context's stream, waited for.  Work already queued on that
+// stream (zero fills of a new basis, products) is ordered before it, which a
+// null-stream hipMemcpy does not give: the context's stream does not wait for
+// the null stream, nor the null stream for it.
+void stream_copy(irlba_hip_context* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    if (bytes == 0) return;
+    IRLBA_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
+    IRLBA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
 std::shared_ptr<double> upload_vec(irlba_hip_context* ctx, const double* src, i64 n, bool device_ptrs) {
     auto d = dalloc<double>(n);
     // The caller's device vector may still be written by work on another stream.
@@ -436,7 +446,7 @@ TiledLines build_lines(irlba_hip_context* ctx, const std::int32_t* d_idx, const 
     auto ptr = dalloc<i64>(nseg + 1);
     exclusive_scan_i64(plen.get(), ptr.get(), nseg, s);
     plen.reset();
-    IRLBA_HIP_CHECK(hipMemcpy(&t.slots, ptr.get() + nseg, sizeof(i64), hipMemcpyDeviceToHost));
+    stream_copy(ctx, &t.slots, ptr.get() + nseg, sizeof(i64), hipMemcpyDeviceToHost);
 
     auto cursor = dalloc<unsigned long long>(nseg);
     IRLBA_HIP_CHECK(hipMemcpyAsync(cursor.get(), ptr.get(), sizeof(i64) * nseg, hipMemcpyDeviceToDevice, s));
@@ -483,7 +493,7 @@ TiledLines build_lines(irlba_hip_context* ctx, const std::int32_t* d_idx, const 
         exclusive_scan_i64(plenB.get(), ptrB.get(), nseg, s);
         plenB.reset();
         i64 slotsB = 0;
-        IRLBA_HIP_CHECK(hipMemcpy(&slotsB, ptrB.get() + nseg, sizeof(i64), hipMemcpyDeviceToHost));
+        stream_copy(ctx, &slotsB, ptrB.get() + nseg, sizeof(i64), hipMemcpyDeviceToHost);
         const double bytes_u16 = static_cast<double>(t.slots) * (vb + 2.0);
         const double bytes_u8 = static_cast<double>(slotsB) * (vb + 1.0);
         if (slotsB > 0 && (bytes_u8 < 0.97 * bytes_u16 || delta_mode == 2)) {
@@ -604,9 +614,9 @@ std::shared_ptr<SparseBase> make_sparse(irlba_hip_context* ctx, i64 nrow, i64 nc
         u_idx = dalloc_bytes(sizeof(std::int32_t) * nnz);
         u_val = dalloc_bytes(storage_bytes(in_t) * nnz);
         u_ptr = dalloc_bytes(sizeof(i64) * (primary + 1));
-        IRLBA_HIP_CHECK(hipMemcpy(u_idx.get(), indices, sizeof(std::int32_t) * nnz, hipMemcpyHostToDevice));
-        IRLBA_HIP_CHECK(hipMemcpy(u_val.get(), values, storage_bytes(in_t) * nnz, hipMemcpyHostToDevice));
-        IRLBA_HIP_CHECK(hipMemcpy(u_ptr.get(), pointers, sizeof(i64) * (primary + 1), hipMemcpyHostToDevice));
+        stream_copy(ctx, u_idx.get(), indices, sizeof(std::int32_t) * nnz, hipMemcpyHostToDevice);
+        stream_copy(ctx, u_val.get(), values, storage_bytes(in_t) * nnz, hipMemcpyHostToDevice);
+        stream_copy(ctx, u_ptr.get(), pointers, sizeof(i64) * (primary + 1), hipMemcpyHostToDevice);
         d_idx = static_cast<const std::int32_t*>(u_idx.get());
         d_val = u_val.get();
         d_ptr = static_cast<const i64*>(u_ptr.get());
@@ -792,7 +802,7 @@ void HandleOp::realize_host(HostMat& out) {
         std::vector<double> h;
         if (v) {
             h.resize(n);
-            IRLBA_HIP_CHECK(hipMemcpy(h.data(), v.get(), sizeof(double) * n, hipMemcpyDeviceToHost));
+            stream_copy(ctx, h.data(), v.get(), sizeof(double) * n, hipMemcpyDeviceToHost);
         }
         return h;
     };
@@ -1599,7 +1609,7 @@ int irlba_hip_dense_create(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, i
         } else {
             // Host input of another type: copied as given, converted on the device.
             auto raw = dalloc_bytes(storage_bytes(in_t) * n);
-            IRLBA_HIP_CHECK(hipMemcpy(raw.get(), values, storage_bytes(in_t) * n, hipMemcpyHostToDevice));
+            stream_copy(ctx, raw.get(), values, storage_bytes(in_t) * n, hipMemcpyHostToDevice);
             launch_convert_values(raw.get(), in_t, d->data.get(), st, n, ctx->stream);
             IRLBA_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         }
@@ -1733,8 +1743,8 @@ int irlba_hip_realize(const irlba_hip_matrix* m, double* buffer, int device_poin
         op.m = m;
         HostMat A;
         op.realize_host(A);
-        IRLBA_HIP_CHECK(hipMemcpy(buffer, A.a.data(), sizeof(double) * A.a.size(),
-                                  device_pointers ? hipMemcpyHostToDevice : hipMemcpyHostToHost));
+        stream_copy(m->ctx, buffer, A.a.data(), sizeof(double) * A.a.size(),
+                    device_pointers ? hipMemcpyHostToDevice : hipMemcpyHostToHost);
     });
 }
 
@@ -1796,15 +1806,16 @@ int irlba_hip_lanczos(const irlba_hip_matrix* m, int64_t work, double* W, double
         o.seed = seed;
         Engine eg(ctx, op, o, work);
         const i64 nr = op.nr, nc = op.nc;
-        IRLBA_HIP_CHECK(hipMemcpy(eg.W.get(), W, sizeof(double) * nr * work, hipMemcpyHostToDevice));
-        IRLBA_HIP_CHECK(hipMemcpy(eg.V.get(), V, sizeof(double) * nc * work, hipMemcpyHostToDevice));
+        // After the engine's zero fills of W and V, which are queued on the same stream.
+        stream_copy(ctx, eg.W.get(), W, sizeof(double) * nr * work, hipMemcpyHostToDevice);
+        stream_copy(ctx, eg.V.get(), V, sizeof(double) * nc * work, hipMemcpyHostToDevice);
         HostMat hb(work, work);
         std::copy(B, B + work * work, hb.a.begin());
         eg.sweep(start, hb);
         std::copy(hb.a.begin(), hb.a.end(), B);
-        IRLBA_HIP_CHECK(hipMemcpy(W, eg.W.get(), sizeof(double) * nr * work, hipMemcpyDeviceToHost));
-        IRLBA_HIP_CHECK(hipMemcpy(V, eg.V.get(), sizeof(double) * nc * work, hipMemcpyDeviceToHost));
-        IRLBA_HIP_CHECK(hipMemcpy(F, eg.F.get(), sizeof(double) * nc, hipMemcpyDeviceToHost));
+        stream_copy(ctx, W, eg.W.get(), sizeof(double) * nr * work, hipMemcpyDeviceToHost);
+        stream_copy(ctx, V, eg.V.get(), sizeof(double) * nc * work, hipMemcpyDeviceToHost);
+        stream_copy(ctx, F, eg.F.get(), sizeof(double) * nc, hipMemcpyDeviceToHost);
     });
 }
 
@@ -1822,8 +1833,8 @@ int irlba_hip_orthogonalize(irlba_hip_context* ctx, const double* M, int64_t row
         auto t = dalloc<double>(std::max<i64>(ncols, 1));
         auto np = dalloc<double>(G);
         auto nrm = dalloc<double>(1);
-        IRLBA_HIP_CHECK(hipMemcpy(dM.get(), M, sizeof(double) * rows * total_cols, hipMemcpyHostToDevice));
-        IRLBA_HIP_CHECK(hipMemcpy(df.get(), vec, sizeof(double) * rows, hipMemcpyHostToDevice));
+        stream_copy(ctx, dM.get(), M, sizeof(double) * rows * total_cols, hipMemcpyHostToDevice);
+        stream_copy(ctx, df.get(), vec, sizeof(double) * rows, hipMemcpyHostToDevice);
         if (ncols > 0) {
             launch_cgs_dots(dM.get(), rows, static_cast<int>(ncols), df.get(), rows, part.get(), s);
             launch_cgs_reduce(part.get(), rows, static_cast<int>(ncols), t.get(), s);
@@ -1833,8 +1844,8 @@ int irlba_hip_orthogonalize(irlba_hip_context* ctx, const double* M, int64_t row
         }
         launch_normalize(df.get(), rows, np.get(), G, nrm.get(), nullptr, s);
         IRLBA_HIP_CHECK(hipStreamSynchronize(s));
-        IRLBA_HIP_CHECK(hipMemcpy(vec, df.get(), sizeof(double) * rows, hipMemcpyDeviceToHost));
-        if (norm_out) IRLBA_HIP_CHECK(hipMemcpy(norm_out, nrm.get(), sizeof(double), hipMemcpyDeviceToHost));
+        stream_copy(ctx, vec, df.get(), sizeof(double) * rows, hipMemcpyDeviceToHost);
+        if (norm_out) stream_copy(ctx, norm_out, nrm.get(), sizeof(double), hipMemcpyDeviceToHost);
     });
 }
 
@@ -1852,12 +1863,12 @@ int irlba_hip_rotate(irlba_hip_context* ctx, const double* M, int64_t rows, int6
         auto dM = dalloc<double>(rows * work);
         auto dY = dalloc<double>(work * ldy);
         auto dO = dalloc<double>(rows * std::max<i64>(k, 1));
-        IRLBA_HIP_CHECK(hipMemcpy(dM.get(), M, sizeof(double) * rows * work, hipMemcpyHostToDevice));
-        IRLBA_HIP_CHECK(hipMemcpy(dY.get(), yt.data(), sizeof(double) * yt.size(), hipMemcpyHostToDevice));
+        stream_copy(ctx, dM.get(), M, sizeof(double) * rows * work, hipMemcpyHostToDevice);
+        stream_copy(ctx, dY.get(), yt.data(), sizeof(double) * yt.size(), hipMemcpyHostToDevice);
         launch_rotate(dM.get(), rows, static_cast<int>(work), dY.get(), ldy, static_cast<int>(k), dO.get(), rows, rows,
                       s);
         IRLBA_HIP_CHECK(hipStreamSynchronize(s));
-        IRLBA_HIP_CHECK(hipMemcpy(out, dO.get(), sizeof(double) * rows * k, hipMemcpyDeviceToHost));
+        stream_copy(ctx, out, dO.get(), sizeof(double) * rows * k, hipMemcpyDeviceToHost);
     });
 }
 
@@ -1950,13 +1961,13 @@ int irlba_hip_generate_csc(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, d
         auto ptr = dalloc<i64>(ncol + 1);
         exclusive_scan_i64(counts.get(), ptr.get(), ncol, s);
         i64 nnz = 0;
-        IRLBA_HIP_CHECK(hipMemcpy(&nnz, ptr.get() + ncol, sizeof(i64), hipMemcpyDeviceToHost));
+        stream_copy(ctx, &nnz, ptr.get() + ncol, sizeof(i64), hipMemcpyDeviceToHost);
         if (nnz_out) *nnz_out = nnz;
         if (!values) return;
         double table[32];
         for (int k = 0; k < 32; ++k) table[k] = std::log1p(static_cast<double>(k));
         auto dtab = dalloc<double>(32);
-        IRLBA_HIP_CHECK(hipMemcpy(dtab.get(), table, sizeof(table), hipMemcpyHostToDevice));
+        stream_copy(ctx, dtab.get(), table, sizeof(table), hipMemcpyHostToDevice);
         launch_gen_nb_fill(nrow, ncol, thr, seed, ptr.get(), dtab.get(), value_type == IRLBA_HIP_F32, values, indices,
                            s);
         IRLBA_HIP_CHECK(hipMemcpyAsync(pointers, ptr.get(), sizeof(i64) * (ncol + 1), hipMemcpyDeviceToDevice, s));

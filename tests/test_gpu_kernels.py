@@ -71,6 +71,32 @@ def test_lanczos_sweep_vs_oracle(nr, nc, work, start):
     np.testing.assert_allclose(hV.T @ hV, np.eye(work), atol=1e-11)
 
 
+def test_lanczos_inputs_land_after_the_zero_fill():
+    # The standalone sweep zero-fills its new bases on the context's stream and
+    # then uploads the caller's W and V; both go through that stream, so the
+    # upload always lands after the fill.  A restart-state sweep on a 400000-row
+    # basis, repeated: every run bit-identical to the first.
+    nr, nc, work = 400000, 3000, 24
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.002, 31, f32=False)
+    H = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr)
+    rng = np.random.default_rng(6)
+    V = np.zeros((nc, work))
+    W = np.zeros((nr, work))
+    B = np.zeros((work, work))
+    V[:, 0] = rng.standard_normal(nc)
+    V[:, 0] /= np.linalg.norm(V[:, 0])
+    w1, v1, b1, f1 = irlba.run_lanczos_bidiagonalization(H, W[:, :1], V[:, :1], B[:1, :1], 50, 0)
+    W[:, :1], V[:, :1] = w1, v1
+    V[:, 1] = f1 / np.linalg.norm(f1)
+    B[0, 0] = b1[0, 0]
+    first = irlba.run_lanczos_bidiagonalization(H, W, V, B, 50, 1)
+    assert np.abs(first[2]).max() > 0
+    for _ in range(8):
+        again = irlba.run_lanczos_bidiagonalization(H, W, V, B, 50, 1)
+        for a, b in zip(again, first):
+            np.testing.assert_array_equal(a, b)
+
+
 def _csc_dense(vals, idx, ptr, nr, nc):
     A = np.zeros((nr, nc))
     for c in range(nc):
