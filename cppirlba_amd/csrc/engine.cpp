@@ -27,6 +27,8 @@
 #include <vector>
 
 #include "../../include/irlba_hip.h"
+#include <mutex>
+
 #include "common.hpp"
 #include "host_svd.hpp"
 
@@ -82,6 +84,11 @@ StorageType to_storage(int t) {
 
 // ------------------------------------------------------------------ context
 struct irlba_hip_context {
+    // Calls on one context are serialised: its stream, workspaces, events and
+    // counters are shared, and the reference lets several threads run
+    // compute() on one Matrix at once (include/irlba/Matrix/interface.hpp:171-185).
+    // Recursive, since some entry points call others.
+    mutable std::recursive_mutex mu;
     int device = 0;
     hipStream_t stream = nullptr;
     int cu_count = 256;
@@ -191,6 +198,14 @@ namespace {
 
 struct DeviceGuard {
     explicit DeviceGuard(int dev) { IRLBA_HIP_CHECK(hipSetDevice(dev)); }
+};
+
+// Holds the context's lock for the duration of an entry point, then makes its
+// device current.
+struct CtxGuard {
+    std::lock_guard<std::recursive_mutex> lk;
+    DeviceGuard dg;
+    explicit CtxGuard(irlba_hip_context* c) : lk(c->mu), dg(c->device) {}
 };
 
 // ------------------------------------------------------------------ storage
@@ -1540,7 +1555,7 @@ int irlba_hip_context_destroy(irlba_hip_context* ctx) {
 int irlba_hip_context_synchronize(irlba_hip_context* ctx) {
     return guarded([&] {
         require(ctx != nullptr, "null context");
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         ctx->sync();
     });
 }
@@ -1548,6 +1563,7 @@ int irlba_hip_context_synchronize(irlba_hip_context* ctx) {
 int irlba_hip_get_stats(const irlba_hip_context* ctx, irlba_hip_stats* out) {
     return guarded([&] {
         require(ctx && out, "null argument");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         *out = ctx->stats;
     });
 }
@@ -1556,6 +1572,7 @@ int irlba_hip_context_set_profiling(irlba_hip_context* ctx, int period) {
     return guarded([&] {
         require(ctx != nullptr, "null context");
         if (period < 0) throw std::runtime_error("profiling period must be non-negative");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         ctx->profiling = period;
         ctx->profile_seq[0] = ctx->profile_seq[1] = 0;
     });
@@ -1568,7 +1585,7 @@ int irlba_hip_sparse_create(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, 
                             int accumulate_f64, int device_pointers, irlba_hip_matrix** out) {
     return guarded([&] {
         require(ctx && out, "null argument");
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         auto base = make_sparse(ctx, nrow, ncol, to_storage(input_type), values, indices, pointers, column_major != 0,
                                 to_storage(storage_type), accumulate_f64 != 0, device_pointers != 0);
         auto* m = new irlba_hip_matrix;
@@ -1583,7 +1600,7 @@ int irlba_hip_dense_create(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, i
     return guarded([&] {
         require(ctx && out, "null argument");
         require(nrow >= 0 && ncol >= 0, "matrix dimensions must be non-negative");
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         const StorageType in_t = to_storage(input_type), st = to_storage(storage_type);
         auto d = std::make_shared<DenseBase>();
         d->ctx = ctx;
@@ -1634,7 +1651,7 @@ int irlba_hip_centered_create(const irlba_hip_matrix* inner, const double* cente
                               irlba_hip_matrix** out) {
     return guarded([&] {
         require(inner && out && center, "null argument");
-        DeviceGuard g(inner->ctx->device);
+        CtxGuard g(inner->ctx);
         *out = decorate_center(inner, upload_vec(inner->ctx, center, inner->base->nc, device_pointers != 0));
     });
 }
@@ -1643,7 +1660,7 @@ int irlba_hip_scaled_create(const irlba_hip_matrix* inner, const double* scale, 
                             int device_pointers, irlba_hip_matrix** out) {
     return guarded([&] {
         require(inner && out && scale, "null argument");
-        DeviceGuard g(inner->ctx->device);
+        CtxGuard g(inner->ctx);
         const i64 n = column ? inner->base->nc : inner->base->nr;
         *out = decorate_scale(inner, upload_vec(inner->ctx, scale, n, device_pointers != 0), column != 0, divide != 0);
     });
@@ -1652,7 +1669,7 @@ int irlba_hip_scaled_create(const irlba_hip_matrix* inner, const double* scale, 
 int irlba_hip_matrix_destroy(irlba_hip_matrix* m) {
     return guarded([&] {
         if (!m) return;
-        DeviceGuard g(m->ctx->device);
+        CtxGuard g(m->ctx);
         IRLBA_HIP_CHECK(hipStreamSynchronize(m->ctx->stream));
         delete m;
     });
@@ -1689,7 +1706,7 @@ int irlba_hip_multiply(const irlba_hip_matrix* m, int adjoint, const double* rig
     return guarded([&] {
         require(m && right && output, "null argument");
         irlba_hip_context* ctx = m->ctx;
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         const i64 nin = adjoint ? m->base->nr : m->base->nc, nout = adjoint ? m->base->nc : m->base->nr;
         if (device_pointers) {
             apply_handle(*m, adjoint != 0, right, output, nullptr, nullptr);
@@ -1711,7 +1728,7 @@ int irlba_hip_multiply_block(const irlba_hip_matrix* m, int adjoint, int64_t k, 
         require(m && right && output, "null argument");
         require(k >= 1 && k <= 128, "the block must have between 1 and 128 columns");
         irlba_hip_context* ctx = m->ctx;
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         const i64 nin = adjoint ? m->base->nr : m->base->nc, nout = adjoint ? m->base->nc : m->base->nr;
         const double* X = right;
         double* Y = output;
@@ -1735,7 +1752,7 @@ int irlba_hip_multiply_block(const irlba_hip_matrix* m, int adjoint, int64_t k, 
 int irlba_hip_realize(const irlba_hip_matrix* m, double* buffer, int device_pointers) {
     return guarded([&] {
         require(m && buffer, "null argument");
-        DeviceGuard g(m->ctx->device);
+        CtxGuard g(m->ctx);
         HandleOp op;
         op.ctx = m->ctx;
         op.nr = m->base->nr;
@@ -1752,7 +1769,7 @@ int irlba_hip_compute(const irlba_hip_matrix* m, int64_t number, const irlba_hip
                       double* D, int64_t* number_out, int32_t* converged, int32_t* iterations, int device_pointers) {
     return guarded([&] {
         require(m != nullptr, "null matrix");
-        DeviceGuard g(m->ctx->device);
+        CtxGuard g(m->ctx);
         HandleOp op;
         op.ctx = m->ctx;
         op.nr = m->base->nr;
@@ -1774,7 +1791,7 @@ int irlba_hip_compute_callback(irlba_hip_context* ctx, int64_t nrow, int64_t nco
                                double* D, int64_t* number_out, int32_t* converged, int32_t* iterations) {
     return guarded([&] {
         require(ctx && matvec, "null argument");
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         CallbackOp op;
         op.ctx = ctx;
         op.nr = nrow;
@@ -1796,7 +1813,7 @@ int irlba_hip_lanczos(const irlba_hip_matrix* m, int64_t work, double* W, double
         require(m && W && V && B && F, "null argument");
         require(work > 0 && start >= 0 && start < work, "start must lie within the working subspace");
         irlba_hip_context* ctx = m->ctx;
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         HandleOp op;
         op.ctx = ctx;
         op.nr = m->base->nr;
@@ -1824,7 +1841,7 @@ int irlba_hip_orthogonalize(irlba_hip_context* ctx, const double* M, int64_t row
     return guarded([&] {
         require(ctx && M && vec, "null argument");
         require(ncols >= 0 && ncols <= total_cols, "ncols must not exceed the columns of the basis");
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         hipStream_t s = ctx->stream;
         auto dM = dalloc<double>(rows * std::max<i64>(total_cols, 1));
         auto df = dalloc<double>(rows);
@@ -1854,7 +1871,7 @@ int irlba_hip_rotate(irlba_hip_context* ctx, const double* M, int64_t rows, int6
     return guarded([&] {
         require(ctx && M && Y && out, "null argument");
         require(k >= 0 && k <= work, "k must not exceed work");
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         hipStream_t s = ctx->stream;
         const int ldy = static_cast<int>((work + 15) / 16 * 16);
         std::vector<double> yt(static_cast<size_t>(work * ldy), 0.0);
@@ -1878,7 +1895,7 @@ int irlba_hip_column_stats(const irlba_hip_matrix* m, int center, int scale, dou
         require(m != nullptr, "null matrix");
         require(!m->decorated, "column statistics are computed on an undecorated matrix");
         irlba_hip_context* ctx = m->ctx;
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         const i64 nr = m->base->nr, nc = m->base->nc;
         if (center && nr < 1) throw std::runtime_error("cannot center with no observations");
         if (scale && nr < 2) throw std::runtime_error("cannot scale with fewer than two observations");
@@ -1898,7 +1915,7 @@ int irlba_hip_pca(const irlba_hip_matrix* m, int center, int scale, int64_t numb
         require(m != nullptr, "null matrix");
         require(!m->decorated, "pca() takes an undecorated matrix");
         irlba_hip_context* ctx = m->ctx;
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         const i64 nr = m->base->nr, nc = m->base->nc;
         // pca.hpp:66-109
         if (center && nr < 1) throw std::runtime_error("cannot center with no observations");
@@ -1951,7 +1968,7 @@ int irlba_hip_generate_csc(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, d
     return guarded([&] {
         require(ctx != nullptr, "null context");
         require(value_type == IRLBA_HIP_F32 || value_type == IRLBA_HIP_F64, "generated values are F32 or F64");
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         hipStream_t s = ctx->stream;
         double t = std::floor(density * 4294967296.0);
         t = std::min(std::max(t, 0.0), 4294967295.0);
@@ -1979,7 +1996,7 @@ int irlba_hip_generate_sparse(irlba_hip_context* ctx, int64_t nrow, int64_t ncol
                               int storage_type, int accumulate_f64, irlba_hip_matrix** out) {
     return guarded([&] {
         require(ctx && out, "null argument");
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         i64 nnz = 0;
         if (irlba_hip_generate_csc(ctx, nrow, ncol, density, seed, IRLBA_HIP_F32, nullptr, nullptr, nullptr, &nnz) !=
             IRLBA_HIP_OK)
@@ -2009,7 +2026,7 @@ int irlba_hip_svd_backward(const irlba_hip_matrix* m, int64_t k, const double* U
         require(complement >= 0 && complement <= 2, "complement must be 0, 1 or 2");
         require(max_cg_iterations >= 0 && cg_tolerance >= 0, "invalid CG settings");
         irlba_hip_context* ctx = m->ctx;
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         const i64 nr = m->base->nr, nc = m->base->nc;
         require(k > 0 && k <= std::min(nr, nc) && k <= 64, "k must lie in [1, min(rows, cols)] and be at most 64");
         // Absent gradients are zero.
@@ -2042,7 +2059,7 @@ int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const in
         require(nrow >= 0 && ncol >= 0, "negative dimension");
         require(rank >= 1 && rank <= 128, "rank must lie in [1, 128]");
         require(out_type == IRLBA_HIP_F64 || out_type == IRLBA_HIP_F32, "out_type must be F64 or F32");
-        DeviceGuard g(ctx->device);
+        CtxGuard g(ctx);
         hipStream_t s = ctx->stream;
         const bool f32 = out_type == IRLBA_HIP_F32;
         const size_t ob = f32 ? 4 : 8;

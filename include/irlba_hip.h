@@ -14,8 +14,13 @@
  * stream, which does not wait for other streams, so their device inputs must
  * be complete when the call is made (e.g. after torch.cuda.synchronize()).
  * One context = one device + one stream; calls on a context are stream
- * ordered and a context must not be used from two threads at once (the
- * single-caller rule of a reference Workspace, include/irlba/Matrix/scaled.hpp:49).
+ * ordered, and calls from several threads on one context (or on handles of
+ * one context) are serialised by a per-context lock, so a matrix handle can be
+ * shared by concurrent compute()/pca() calls as a reference Matrix can
+ * (include/irlba/Matrix/interface.hpp:171-185).  Threads that need to run at
+ * the same time use one context each.  A product callback runs with its
+ * context's lock held: it may call back into the same context on its own
+ * thread, but must not wait on another thread that uses that context.
  * A matrix handle must not outlive its context; decorator handles share the
  * storage of the handle they wrap (include/irlba/Matrix/centered.hpp:139-159).
  */

@@ -727,6 +727,30 @@ def test_failing_callback_leaves_context_usable():
     assert_equal_vectors(res.D, ref.D, 1e-10)
 
 
+def test_concurrent_solves_on_one_matrix():
+    # A Matrix is const and shareable across concurrent compute() calls
+    # (interface.hpp:171-185): threads sharing one device handle (one context)
+    # get the serial results bit for bit; the context serialises their calls.
+    from concurrent.futures import ThreadPoolExecutor
+
+    vals, idx, ptr = O.gen_nb_sparse(3000, 2000, 0.05, 13, f32=False)
+    H = irlba.HipSparseMatrix(3000, 2000, vals, idx, ptr)
+    D = irlba.HipDenseMatrix(O.create_random_matrix(400, 300))
+    inits = [O.create_random_vector(2000, s) for s in range(4)]
+
+    def job(i):
+        r1 = irlba.compute(H, 6, irlba.Options(initial=inits[i % 4]))
+        r2 = irlba.pca(D, True, True, 4, irlba.Options(initial=O.create_random_vector(300, i % 4)))
+        return r1.D, r1.U, r2.variances, r2.scores
+
+    serial = [job(i) for i in range(4)]
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        concurrent = list(ex.map(job, range(12)))
+    for i, got in enumerate(concurrent):
+        for a, b in zip(got, serial[i % 4]):
+            np.testing.assert_array_equal(a, b)
+
+
 def test_sampled_product_timing_and_device_results():
     # Context.set_profiling(period): one product in `period` of each direction
     # carries HIP events; the counts and the results do not depend on it.
