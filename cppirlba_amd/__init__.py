@@ -16,6 +16,7 @@ through libirlba_hip.so; the package has no CPU fallback.
 """
 
 import ctypes
+import threading
 from dataclasses import dataclass, field
 from typing import Optional
 
@@ -133,12 +134,16 @@ class Context:
 
 
 _default = None
+_default_lock = threading.Lock()
 
 
 def default_context():
+    """The shared context on device 0, created once even when several threads ask first."""
     global _default
     if _default is None:
-        _default = Context(0)
+        with _default_lock:
+            if _default is None:
+                _default = Context(0)
     return _default
 
 
