@@ -147,9 +147,13 @@ def default_context():
     return _default
 
 
+def _is_tensor(a):
+    return hasattr(a, "data_ptr") and hasattr(a, "is_cuda")
+
+
 def _ptr(a):
     """(pointer, is_device, keepalive) for a numpy array or a torch tensor."""
-    if hasattr(a, "data_ptr") and hasattr(a, "is_cuda"):
+    if _is_tensor(a):
         if a.is_cuda:
             import torch
 
@@ -157,6 +161,58 @@ def _ptr(a):
             return ctypes.c_void_p(a.data_ptr()), True, a
         a = a.numpy()
     return ctypes.c_void_p(a.ctypes.data), False, a
+
+
+def _dtype_name(a):
+    return str(a.dtype).replace("torch.", "")
+
+
+def _sparse_values(x):
+    """The values of a compressed matrix as float64 or float32 (the engine's two
+    input types): numpy arrays and torch tensors, host or device.  Half and
+    bfloat16 widen to float32 and integer types to float64, both exactly;
+    anything else (complex, bool) is rejected."""
+    name = _dtype_name(x) if hasattr(x, "dtype") else "float64"
+    if "complex" in name or name == "bool":
+        raise TypeError(f"sparse values must be real numbers, not {name}")
+    if _is_tensor(x):
+        import torch
+
+        if name in ("float16", "bfloat16"):
+            x = x.to(torch.float32)
+        elif name not in ("float32", "float64"):
+            x = x.to(torch.float64)
+        return x.contiguous(), (F32 if x.dtype == torch.float32 else F64)
+    x = np.asarray(x)
+    if x.dtype == np.float32:
+        return np.ascontiguousarray(x), F32
+    if x.dtype == np.float16:
+        return np.ascontiguousarray(x, dtype=np.float32), F32
+    return np.ascontiguousarray(x, dtype=np.float64), F64
+
+
+def _int_array(a, want, what):
+    """A contiguous int32 (indices) or int64 (pointers) array from a numpy array
+    or torch tensor; integer inputs of another width are converted after a
+    range check, anything else raises."""
+    name = _dtype_name(a) if hasattr(a, "dtype") else "int64"
+    if "int" not in name or name == "bool":
+        raise TypeError(f"sparse {what} must be integers, not {name}")
+    lim = 2 ** 31 if want == "int32" else 2 ** 63
+    if _is_tensor(a):
+        import torch
+
+        tdt = torch.int32 if want == "int32" else torch.int64
+        if a.dtype != tdt:
+            if a.numel() and (int(a.max()) >= lim or int(a.min()) < -lim):
+                raise ValueError(f"sparse {what} do not fit in {want}")
+            a = a.to(tdt)
+        return a.contiguous()
+    a = np.asarray(a)
+    if a.dtype != np.dtype(want):
+        if a.size and (int(a.max()) >= lim or int(a.min()) < -lim):
+            raise ValueError(f"sparse {what} do not fit in {want}")
+    return np.ascontiguousarray(a, dtype=want)
 
 
 class _Handle:
@@ -231,31 +287,59 @@ class _Handle:
 
 
 class HipSparseMatrix(_Handle):
-    """Device drop-in for ParallelSparseMatrix(nrow, ncol, x, i, p, column_major, num_threads).
+    """Device drop-in for ParallelSparseMatrix(nrow, ncol, x, i, p, column_major, num_threads)
+    (Matrix/sparse.hpp:438-472), same positional arguments.
 
+    ``num_threads`` is accepted for the reference's signature; the products run
+    on the GPU, whose grid replaces the thread partition.  ``x`` (values), ``i``
+    (indices) and ``p`` (pointers) are numpy arrays or torch tensors, all on the
+    host or all on the device; values are taken as float64 or float32 (half
+    precision widens to float32, integers to float64), indices as int32 and
+    pointers as int64 (other integer widths are converted after a range check).
     ``storage``: value type kept in HBM ("f64" or "f32"; default: the input's).
     ``accumulate``: "f64" or "f32" (mixed precision); default "f64" for FP64
     storage and "f32" for FP32 storage.
     """
 
-    def __init__(self, nrow, ncol, x, i, p, column_major=True, storage=None, accumulate=None, ctx=None):
-        ctx = ctx or default_context()
-        xp, xdev, xk = _ptr(x if hasattr(x, "data_ptr") else np.ascontiguousarray(x))
-        dtype = str(getattr(xk, "dtype", "float64")).replace("torch.", "")
-        in_t = F32 if "float32" in dtype else F64
-        if not xdev and in_t == F64:
-            xk = np.ascontiguousarray(xk, dtype=np.float64)
-            xp = ctypes.c_void_p(xk.ctypes.data)
-        ip, idev, ik = _ptr(i if hasattr(i, "data_ptr") else np.ascontiguousarray(i, dtype=np.int32))
-        pp, pdev, pk = _ptr(p if hasattr(p, "data_ptr") else np.ascontiguousarray(p, dtype=np.int64))
+    def __init__(self, nrow, ncol, x, i, p, column_major=True, num_threads=1, *, storage=None, accumulate=None,
+                 ctx=None):
+        if int(num_threads) < 1:
+            raise ValueError("num_threads must be positive")
+        x, in_t = _sparse_values(x)
+        i = _int_array(i, "int32", "indices")
+        p = _int_array(p, "int64", "pointers")
+        primary = ncol if column_major else nrow
+        if p.ndim != 1 or p.shape[0] != primary + 1:
+            raise ValueError("pointers must have one entry per column (or row) plus one")
+        xp, xdev, xk = _ptr(x)
+        ip, idev, ik = _ptr(i)
+        pp, pdev, pk = _ptr(p)
         if not (xdev == idev == pdev):
             raise ValueError("values, indices and pointers must all be on the host or all on the device")
+        if xk.shape[0] != ik.shape[0]:
+            raise ValueError("values and indices must have the same length")
         st = _type_code(storage) if storage is not None else in_t
         acc = _type_code(accumulate) if accumulate is not None else (F64 if st == F64 else F32)
+        ctx = ctx or default_context()
         h = ctypes.c_void_p()
         check(lib().irlba_hip_sparse_create(ctx.handle, nrow, ncol, in_t, xp, ip, pp, int(column_major), st,
                                             int(acc == F64), int(xdev), ctypes.byref(h)))
         super().__init__(ctx, h)
+        self._x, self._i, self._p = xk, ik, pk
+        self.column_major = bool(column_major)
+        self.num_threads = int(num_threads)
+
+    def get_values(self):
+        """The values as given to the constructor (sparse.hpp:495-501)."""
+        return self._x
+
+    def get_indices(self):
+        """The indices as given to the constructor (sparse.hpp:503-510)."""
+        return self._i
+
+    def get_pointers(self):
+        """The pointers as given to the constructor (sparse.hpp:512-517)."""
+        return self._p
 
 
 
@@ -266,18 +350,28 @@ class HipDenseMatrix(_Handle):
     """
 
     def __init__(self, array, storage="f64", ctx=None):
-        ctx = ctx or default_context()
-        if hasattr(array, "data_ptr") and getattr(array, "is_cuda", False):
+        if _is_tensor(array):
+            import torch
+
+            dt = _dtype_name(array)
+            if "complex" in dt or dt == "bool":
+                raise TypeError(f"dense values must be real numbers, not {dt}")
+            if dt == "float16" or (dt == "bfloat16" and not array.is_cuda):
+                array = array.to(torch.float32)  # exact
+            elif dt not in ("float64", "float32", "bfloat16"):
+                array = array.to(torch.float64)  # integers, exact up to 2^53
+        if _is_tensor(array) and array.is_cuda:
             nr, nc = array.shape
             col_major = not array.is_contiguous()
             if col_major and not array.t().is_contiguous():
                 array = array.contiguous()
                 col_major = False
-            dt = str(array.dtype)
-            in_t = F64 if "float64" in dt else (F32 if "float32" in dt else BF16)
+            in_t = {"float64": F64, "float32": F32, "bfloat16": BF16}[_dtype_name(array)]
             ptr, dev, keep = ctypes.c_void_p(array.data_ptr()), 1, array
         else:
-            a = np.asarray(array)
+            a = array.numpy() if _is_tensor(array) else np.asarray(array)
+            if a.dtype.kind not in "fiu":
+                raise TypeError(f"dense values must be real numbers, not {a.dtype}")
             nr, nc = a.shape
             if a.flags.f_contiguous and not a.flags.c_contiguous:
                 col_major = True
@@ -290,6 +384,7 @@ class HipDenseMatrix(_Handle):
                 a = a.astype(np.float64, copy=False)
                 in_t = F64
             ptr, dev, keep = ctypes.c_void_p(a.ctypes.data), 0, a
+        ctx = ctx or default_context()
         h = ctypes.c_void_p()
         check(lib().irlba_hip_dense_create(ctx.handle, nr, nc, in_t, ptr, int(col_major), _type_code(storage), dev,
                                            ctypes.byref(h)))
@@ -570,17 +665,23 @@ def sddmm(nrow, ncol, indices, pointers, X, Y, column_major=True, out_type="f64"
 
     ``out_type`` "f64" or "f32" (FP32 factors, FP64 accumulation, FP32 output).
     """
-    ctx = ctx or default_context()
     X = _colmajor(X)
     Y = _colmajor(Y)
     if X.shape[0] != nrow or Y.shape[0] != ncol or X.shape[1] != Y.shape[1]:
         raise ValueError("factor shapes do not match the pattern")
-    idx = np.ascontiguousarray(indices, dtype=np.int32)
-    ptr = np.ascontiguousarray(pointers, dtype=np.int64)
-    if ptr.size != (ncol if column_major else nrow) + 1:
+    idx = _int_array(indices, "int32", "indices")
+    ptr = _int_array(pointers, "int64", "pointers")
+    if _is_tensor(idx) or _is_tensor(ptr):
+        idx, ptr = np.asarray(idx.cpu() if _is_tensor(idx) else idx), np.asarray(ptr.cpu() if _is_tensor(ptr) else ptr)
+    if ptr.ndim != 1 or ptr.size != (ncol if column_major else nrow) + 1:
         raise ValueError("pointers have the wrong length")
+    # The output has one slot per stored index: the pattern must address exactly
+    # those, or the kernel would read past the indices and write past the output.
+    if ptr[0] != 0 or ptr[-1] != idx.size or (ptr.size > 1 and np.any(np.diff(ptr) < 0)):
+        raise ValueError("sparse pointers must start at 0, be non-decreasing and end at the number of indices")
     code = _type_code(out_type)
     out = np.zeros(idx.size, dtype=np.float32 if code == F32 else np.float64)
+    ctx = ctx or default_context()
     check(lib().irlba_hip_sddmm(ctx.handle, nrow, ncol, idx.ctypes.data, ptr.ctypes.data, int(column_major),
                                 X.shape[1], X.ctypes.data, Y.ctypes.data, out.ctypes.data, code, 0))
     return out

@@ -1030,10 +1030,29 @@ struct Engine {
         IRLBA_HIP_CHECK(hipStreamWaitEvent(s, ctx->copy_done, 0));
     }
     // out[:, :k] = M[:, :work] Y[:, :k] with the staged slot's Y (out == M allowed).
+    // The rotation kernels work in place for up to 128 output columns (every
+    // row is read before any of its outputs is written); a wider in-place
+    // restart (number + extra_work above ~130) rotates into a scratch basis,
+    // as the reference does with Vtmp / Wtmp (compute.hpp:268,279), and
+    // copies it back in stream order.
     void rotate_staged(int slot, double* M, i64 L, i64 k, double* out, i64 ld_out) {
         const double* dY = Yt.get() + static_cast<i64>(slot) * work * ldy;
+        if (out == M && k > kRotateInPlaceMax) {
+            if (rot_tmp_len < L * k) {
+                IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+                rot_tmp = dalloc<double>(L * k);
+                rot_tmp_len = L * k;
+            }
+            launch_rotate(M, L, static_cast<int>(work), dY, ldy, static_cast<int>(k), rot_tmp.get(), L, L, s);
+            IRLBA_HIP_CHECK(hipMemcpy2DAsync(out, sizeof(double) * ld_out, rot_tmp.get(), sizeof(double) * L,
+                                             sizeof(double) * L, k, hipMemcpyDeviceToDevice, s));
+            return;
+        }
         launch_rotate(M, L, static_cast<int>(work), dY, ldy, static_cast<int>(k), out, ld_out, L, s);
     }
+    static constexpr i64 kRotateInPlaceMax = 128;
+    std::shared_ptr<double> rot_tmp;
+    i64 rot_tmp_len = 0;
 
     double* yt_host = nullptr;
     double* norms_host = nullptr;  // pinned, so the norms copy is asynchronous
@@ -1067,6 +1086,10 @@ std::pair<bool, int> run_compute(irlba_hip_context* ctx, Op& op, i64 number, con
     const auto t0 = std::chrono::steady_clock::now();
     const i64 nr = op.nr, nc = op.nc;
     const i64 smaller = std::min(nr, nc);
+    // The reference takes `number` as an Eigen::Index and sizes its outputs
+    // from it; a negative count has no meaning there and would size the
+    // output copies here from a wrapped value.
+    if (number < 0) throw std::runtime_error("requested number of singular values must be non-negative");
     i64 requested = number;
     if (requested > smaller) {
         if (o.cap_number) requested = smaller;
@@ -1768,6 +1791,7 @@ int irlba_hip_realize(const irlba_hip_matrix* m, double* buffer, int device_poin
 int irlba_hip_compute(const irlba_hip_matrix* m, int64_t number, const irlba_hip_options* opt, double* U, double* V,
                       double* D, int64_t* number_out, int32_t* converged, int32_t* iterations, int device_pointers) {
     return guarded([&] {
+        require(number >= 0, "requested number of singular values must be non-negative");
         require(m != nullptr, "null matrix");
         CtxGuard g(m->ctx);
         HandleOp op;
@@ -1790,6 +1814,7 @@ int irlba_hip_compute_callback(irlba_hip_context* ctx, int64_t nrow, int64_t nco
                                void* user, int64_t number, const irlba_hip_options* opt, double* U, double* V,
                                double* D, int64_t* number_out, int32_t* converged, int32_t* iterations) {
     return guarded([&] {
+        require(number >= 0, "requested number of singular values must be non-negative");
         require(ctx && matvec, "null argument");
         CtxGuard g(ctx);
         CallbackOp op;
@@ -1912,6 +1937,7 @@ int irlba_hip_pca(const irlba_hip_matrix* m, int center, int scale, int64_t numb
                   double* scores, double* rotation, double* variances, int64_t* number_out, int32_t* converged,
                   int32_t* iterations, int device_pointers) {
     return guarded([&] {
+        require(number >= 0, "requested number of singular values must be non-negative");
         require(m != nullptr, "null matrix");
         require(!m->decorated, "pca() takes an undecorated matrix");
         irlba_hip_context* ctx = m->ctx;

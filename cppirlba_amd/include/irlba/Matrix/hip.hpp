@@ -7,10 +7,15 @@
 // The arrays are copied to HBM once; products, Gram-Schmidt and restart
 // rotations then run on the GPU (libirlba_hip.so, include/irlba_hip.h).
 
+#include <cstddef>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "../../../../include/irlba_hip.h"
 #include "interface.hpp"
@@ -46,6 +51,11 @@ inline HipContext& default_hip_context() {
 }
 
 namespace detail {
+
+template <class A, class = void>
+struct has_data : std::false_type {};
+template <class A>
+struct has_data<A, std::void_t<decltype(std::declval<const A&>().data())> > : std::true_type {};
 
 struct HandleDeleter {
     void operator()(irlba_hip_matrix* m) const { irlba_hip_matrix_destroy(m); }
@@ -136,20 +146,61 @@ enum class HipPrecision {
     BF16         // dense only: BF16 values, FP32 products
 };
 
-// Drop-in for ParallelSparseMatrix(nrow, ncol, x, i, p, column_major, num_threads):
-// `x` holds double or float values, `i` int32 indices, `p` int64 (or size_t) pointers.
-template <class EigenVector_ = VectorXd, class EigenMatrix_ = MatrixXd>
+namespace detail {
+
+// A contiguous array of exactly type T holding a[0, n): the caller's own
+// storage when it already is one (anything with data() of element type T),
+// otherwise a converted copy in `tmp`.
+template <typename T, class Array_>
+const T* as_contiguous(const Array_& a, std::size_t n, std::vector<T>& tmp) {
+    typedef typename std::decay<decltype(a[0])>::type E;
+    if constexpr (std::is_same<E, T>::value && has_data<Array_>::value) {
+        return a.data();
+    } else {
+        tmp.resize(n);
+        for (std::size_t j = 0; j < n; ++j) tmp[j] = static_cast<T>(a[j]);
+        return tmp.data();
+    }
+}
+
+}  // namespace detail
+
+// Drop-in for ParallelSparseMatrix<EigenVector_, EigenMatrix_, ValueArray_,
+// IndexArray_, PointerArray_> (include/irlba/Matrix/sparse.hpp:438-589): the
+// same template parameters, the same constructor
+//     (nrow, ncol, x, i, p, column_major, num_threads)
+// (sparse.hpp:469-471), which keeps x, i and p by value as the reference does,
+// and the same getters (sparse.hpp:480-552).  Two trailing arguments, with
+// defaults, choose the device precision and context.
+//
+// The products run on the GPU from device copies of both orientations, built
+// at construction: `num_threads` does not change them.  The reference's
+// per-thread partition of the primary and secondary dimensions (sparse.hpp:
+// 52-114) is still available from the getters for num_threads > 1; it is
+// computed on the host on first request (these getters are documented by the
+// reference as uninitialised when num_threads <= 1, and return empty vectors
+// then).
+//
+// Values of any arithmetic type are accepted: float and double go to the
+// device as they are, other types (e.g. integer counts) as double.  Indices go
+// as int32 and pointers as int64; other integer types are converted on the host.
+template <class EigenVector_ = VectorXd, class EigenMatrix_ = MatrixXd, class ValueArray_ = std::vector<double>,
+          class IndexArray_ = std::vector<int>, class PointerArray_ = std::vector<std::size_t> >
 class HipSparseMatrix final : public HipMatrixBase<EigenVector_, EigenMatrix_> {
 public:
-    template <class ValueArray_, class IndexArray_, class PointerArray_>
-    HipSparseMatrix(Index nrow, Index ncol, const ValueArray_& x, const IndexArray_& i, const PointerArray_& p,
-                    bool column_major, HipPrecision precision = HipPrecision::FP64,
-                    HipContext& ctx = default_hip_context()) {
-        typedef typename std::decay<decltype(x[0])>::type V;
-        static_assert(std::is_same<V, double>::value || std::is_same<V, float>::value, "values must be float or double");
-        static_assert(sizeof(i[0]) == 4, "indices must be 32-bit");
-        static_assert(sizeof(p[0]) == 8, "pointers must be 64-bit");
-        const int in_t = std::is_same<V, double>::value ? IRLBA_HIP_F64 : IRLBA_HIP_F32;
+    typedef typename std::decay<decltype(std::declval<PointerArray_>()[0])>::type PointerType;
+
+    // sparse.hpp:448-451: a default-constructed matrix cannot be used for any operation.
+    HipSparseMatrix() {}
+
+    HipSparseMatrix(Index nrow, Index ncol, ValueArray_ x, IndexArray_ i, PointerArray_ p, bool column_major,
+                    int num_threads = 1, HipPrecision precision = HipPrecision::FP64,
+                    HipContext& ctx = default_hip_context())
+        : my_values(std::move(x)), my_indices(std::move(i)), my_ptrs(std::move(p)), my_column_major(column_major),
+          my_num_threads(num_threads), my_primary(column_major ? ncol : nrow),
+          my_secondary(column_major ? nrow : ncol) {
+        typedef typename std::decay<decltype(my_values[0])>::type V;
+        static_assert(std::is_arithmetic<V>::value, "values must be of an arithmetic type");
         int st = IRLBA_HIP_F64, acc64 = 1;
         if (precision == HipPrecision::MIXED) {
             st = IRLBA_HIP_F32;
@@ -159,11 +210,126 @@ public:
         } else if (precision == HipPrecision::BF16) {
             throw std::runtime_error("BF16 storage is available for dense matrices only");
         }
+        if (nrow < 0 || ncol < 0) throw std::runtime_error("matrix dimensions must be non-negative");
+        std::vector<std::int64_t> ptmp;
+        const std::int64_t* pp =
+            detail::as_contiguous<std::int64_t>(my_ptrs, static_cast<std::size_t>(my_primary) + 1, ptmp);
+        const std::int64_t nnz = pp[my_primary];
+        if (nnz < 0) throw std::runtime_error("sparse pointers must end at a non-negative count");
+        std::vector<std::int32_t> itmp;
+        const std::int32_t* ip = nullptr;
+        typedef typename std::decay<decltype(my_indices[0])>::type IT;
+        if constexpr (sizeof(IT) > 4 || (sizeof(IT) == 4 && std::is_unsigned<IT>::value)) {
+            // Narrowed on the host: out-of-range indices must fail, not wrap into range.
+            itmp.resize(static_cast<std::size_t>(nnz));
+            for (std::int64_t e = 0; e < nnz; ++e) {
+                const IT v = my_indices[e];
+                bool negative = false;
+                if constexpr (std::is_signed<IT>::value) negative = v < 0;
+                if (negative || static_cast<std::uint64_t>(v) >= static_cast<std::uint64_t>(my_secondary))
+                    throw std::runtime_error("sparse indices must lie within the secondary dimension");
+                itmp[e] = static_cast<std::int32_t>(v);
+            }
+            ip = itmp.data();
+        } else {
+            ip = detail::as_contiguous<std::int32_t>(my_indices, static_cast<std::size_t>(nnz), itmp);
+        }
         irlba_hip_matrix* h = nullptr;
-        hip_check(irlba_hip_sparse_create(ctx.get(), nrow, ncol, in_t, &x[0], reinterpret_cast<const int32_t*>(&i[0]),
-                                          reinterpret_cast<const int64_t*>(&p[0]), column_major ? 1 : 0, st, acc64, 0,
-                                          &h));
+        if constexpr (std::is_same<V, float>::value) {
+            std::vector<float> xtmp;
+            const float* xp = detail::as_contiguous<float>(my_values, static_cast<std::size_t>(nnz), xtmp);
+            hip_check(irlba_hip_sparse_create(ctx.get(), nrow, ncol, IRLBA_HIP_F32, xp, ip, pp, column_major ? 1 : 0,
+                                              st, acc64, 0, &h));
+        } else {
+            std::vector<double> xtmp;
+            const double* xp = detail::as_contiguous<double>(my_values, static_cast<std::size_t>(nnz), xtmp);
+            hip_check(irlba_hip_sparse_create(ctx.get(), nrow, ncol, IRLBA_HIP_F64, xp, ip, pp, column_major ? 1 : 0,
+                                              st, acc64, 0, &h));
+        }
         this->adopt(detail::own(h));
+    }
+
+    // sparse.hpp:495-517
+    const ValueArray_& get_values() const { return my_values; }
+    const IndexArray_& get_indices() const { return my_indices; }
+    const PointerArray_& get_pointers() const { return my_ptrs; }
+
+    // sparse.hpp:527-552: the reference's nnz-balanced thread partition.
+    const std::vector<Index>& get_primary_boundaries() const {
+        partition();
+        return my_part->primary;
+    }
+    const std::vector<Index>& get_secondary_boundaries() const {
+        partition();
+        return my_part->secondary;
+    }
+    const std::vector<std::vector<PointerType> >& get_secondary_nonzero_boundaries() const {
+        partition();
+        return my_part->nonzero;
+    }
+
+private:
+    ValueArray_ my_values;
+    IndexArray_ my_indices;
+    PointerArray_ my_ptrs;
+    bool my_column_major = true;
+    int my_num_threads = 1;
+    Index my_primary = 0, my_secondary = 0;
+
+    struct Partition {
+        std::vector<Index> primary, secondary;
+        std::vector<std::vector<PointerType> > nonzero;
+    };
+    mutable std::unique_ptr<Partition> my_part;
+    mutable std::once_flag my_part_once;
+
+    // The partition of sparse.hpp:52-114: thread t gets an equal share of the
+    // non-zeros (the first total % T threads one more), as a range of primary
+    // elements and as a range of secondary elements, plus, per primary element,
+    // the sub-range of its non-zeros whose secondary index falls in each
+    // thread's secondary range.
+    void partition() const {
+        std::call_once(my_part_once, [this] {
+            auto part = std::make_unique<Partition>();
+            const int T = my_num_threads;
+            if (T > 1) {
+                const PointerType total = my_ptrs[my_primary];
+                const PointerType floor_ = total / T;
+                const PointerType extra = total % T;
+                part->primary.assign(static_cast<std::size_t>(T) + 1, 0);
+                Index pc = 0;
+                PointerType sofar = 0;
+                for (int t = 0; t < T; ++t) {
+                    sofar += floor_ + (static_cast<PointerType>(t) < extra ? 1 : 0);
+                    while (pc < my_primary && my_ptrs[pc + 1] <= sofar) ++pc;
+                    part->primary[t + 1] = pc;
+                }
+                std::vector<PointerType> per(static_cast<std::size_t>(my_secondary), 0);
+                for (PointerType e = 0; e < total; ++e) ++per[static_cast<std::size_t>(my_indices[e])];
+                part->secondary.assign(static_cast<std::size_t>(T) + 1, 0);
+                Index sc = 0;
+                PointerType cum = 0;
+                sofar = 0;
+                for (int t = 0; t < T; ++t) {
+                    sofar += floor_ + (static_cast<PointerType>(t) < extra ? 1 : 0);
+                    while (sc < my_secondary && cum <= sofar) cum += per[static_cast<std::size_t>(sc++)];
+                    part->secondary[t + 1] = sc;
+                }
+                part->nonzero.assign(static_cast<std::size_t>(T) + 1,
+                                     std::vector<PointerType>(static_cast<std::size_t>(my_primary)));
+                for (Index c = 0; c < my_primary; ++c) {
+                    const PointerType end = my_ptrs[c + 1];
+                    PointerType e = my_ptrs[c];
+                    part->nonzero[0][c] = e;
+                    for (int t = 0; t < T; ++t) {
+                        const Index lim = part->secondary[t + 1];
+                        while (e < end && static_cast<Index>(my_indices[e]) < lim) ++e;
+                        part->nonzero[t + 1][c] = e;
+                    }
+                }
+            }
+            my_part = std::move(part);
+        });
     }
 };
 

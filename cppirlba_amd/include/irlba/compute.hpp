@@ -8,6 +8,7 @@
 // restart rotations run on the GPU (irlba_hip_compute_callback).
 
 #include <algorithm>
+#include <exception>
 #include <memory>
 #include <stdexcept>
 #include <utility>
@@ -42,12 +43,17 @@ struct COptions {
 };
 
 // Host products of a caller-defined Matrix, called back from the GPU engine.
+// An exception thrown by the caller's workspace cannot cross the C ABI, so it
+// is kept here and rethrown by compute() once the engine has returned: the
+// caller sees its own exception, as when the reference calls the workspace
+// directly (lanczos.hpp:74,93,115).
 template <class EigenVector_, class Matrix_>
 struct HostProducts {
     const Matrix_& matrix;
     std::unique_ptr<Workspace<EigenVector_> > fwd;
     std::unique_ptr<AdjointWorkspace<EigenVector_> > adj;
     EigenVector_ in, out;
+    std::exception_ptr error;
     explicit HostProducts(const Matrix_& m) : matrix(m), fwd(m.new_workspace()), adj(m.new_adjoint_workspace()) {}
 
     static int call(void* user, int adjoint, const double* right, double* output) {
@@ -63,6 +69,7 @@ struct HostProducts {
             std::copy(self->out.data(), self->out.data() + nout, output);
             return 0;
         } catch (...) {
+            if (!self->error) self->error = std::current_exception();
             return 1;
         }
     }
@@ -89,9 +96,11 @@ std::pair<bool, int> compute(const Matrix_& matrix, Index number, EigenMatrix_& 
                                     &conv, &iters, 0));
     } else {
         detail::HostProducts<EigenVector_, Matrix_> hp(matrix);
-        hip_check(irlba_hip_compute_callback(default_hip_context().get(), nr, nc,
-                                             &detail::HostProducts<EigenVector_, Matrix_>::call, &hp, number, &co.o,
-                                             outU.data(), outV.data(), outD.data(), &nout, &conv, &iters));
+        const int rc = irlba_hip_compute_callback(default_hip_context().get(), nr, nc,
+                                                  &detail::HostProducts<EigenVector_, Matrix_>::call, &hp, number,
+                                                  &co.o, outU.data(), outV.data(), outD.data(), &nout, &conv, &iters);
+        if (hp.error) std::rethrow_exception(hp.error);
+        hip_check(rc);
     }
     outU.resize(nr, nout);
     outV.resize(nc, nout);

@@ -5,7 +5,9 @@
 
 #include <cmath>
 #include <cstdio>
+#include <memory>
 #include <random>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -75,6 +77,41 @@ private:
     irlba::MatrixXd A;
 };
 
+// A user exception type, to check that compute() rethrows the object itself.
+struct UserError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// Host dense matrix whose forward product throws on its `fail_at`-th call.
+class ThrowingDense final : public irlba::Matrix<irlba::VectorXd, irlba::MatrixXd> {
+public:
+    ThrowingDense(const irlba::MatrixXd& a, int fail_at) : inner(a), fail(fail_at) {}
+    irlba::Index rows() const { return inner.rows(); }
+    irlba::Index cols() const { return inner.cols(); }
+    struct Fwd final : irlba::Workspace<irlba::VectorXd> {
+        std::unique_ptr<irlba::Workspace<irlba::VectorXd> > base;
+        int fail, calls = 0;
+        Fwd(std::unique_ptr<irlba::Workspace<irlba::VectorXd> > b, int f) : base(std::move(b)), fail(f) {}
+        void multiply(const irlba::VectorXd& x, irlba::VectorXd& y) {
+            if (++calls == fail) throw UserError("user product failed after " + std::to_string(fail) + " calls");
+            base->multiply(x, y);
+        }
+    };
+    std::unique_ptr<irlba::Workspace<irlba::VectorXd> > new_workspace() const {
+        return std::make_unique<Fwd>(inner.new_workspace(), fail);
+    }
+    std::unique_ptr<irlba::AdjointWorkspace<irlba::VectorXd> > new_adjoint_workspace() const {
+        return inner.new_adjoint_workspace();
+    }
+    std::unique_ptr<irlba::RealizeWorkspace<irlba::MatrixXd> > new_realize_workspace() const {
+        return inner.new_realize_workspace();
+    }
+
+private:
+    HostDense inner;
+    int fail;
+};
+
 }  // namespace
 
 int main() {
@@ -99,10 +136,11 @@ int main() {
     for (irlba::Index i = 0; ok && i < 6; ++i) ok = same_same(r1.D[i], r2.D[i], 1e-10);
     check(ok && r1.converged && r2.converged, "compute: HipDenseMatrix == host Matrix subclass");
 
-    // Sparse (CSC) vs dense of the same values.
+    // Sparse (CSC) vs dense of the same values, with the reference's own array
+    // types and constructor (tests/src/sparse.h:11-16; sparse.hpp:469).
     std::vector<double> x;
-    std::vector<int32_t> idx;
-    std::vector<int64_t> ptr(nc + 1, 0);
+    std::vector<int> idx;
+    std::vector<size_t> ptr(nc + 1, 0);
     for (irlba::Index c = 0; c < nc; ++c) {
         for (irlba::Index r = 0; r < nr; ++r)
             if ((r * 7 + c * 3) % 5 == 0) {
@@ -113,14 +151,83 @@ int main() {
     }
     irlba::MatrixXd S(nr, nc);
     for (irlba::Index c = 0; c < nc; ++c)
-        for (int64_t e = ptr[c]; e < ptr[c + 1]; ++e) S(idx[e], c) = x[e];
-    irlba::HipSparseMatrix<> sparse(nr, nc, x, idx, ptr, true);
+        for (size_t e = ptr[c]; e < ptr[c + 1]; ++e) S(idx[e], c) = x[e];
+    const int num_threads = 3;
+    irlba::HipSparseMatrix<irlba::VectorXd, irlba::MatrixXd, std::vector<double>, std::vector<int>, std::vector<size_t> >
+        sparse(nr, nc, x, idx, ptr, true, num_threads);
     irlba::HipDenseMatrix<> sdense(S);
     auto r3 = irlba::compute(sparse, 5, opt);
     auto r4 = irlba::compute(sdense, 5, opt);
     ok = true;
     for (irlba::Index i = 0; i < 5; ++i) ok = ok && same_same(r3.D[i], r4.D[i], 1e-8);
     check(ok, "compute: HipSparseMatrix == HipDenseMatrix");
+    check(sparse.get_values() == x && sparse.get_indices() == idx && sparse.get_pointers() == ptr,
+          "HipSparseMatrix getters return the constructor's arrays");
+
+    // The thread partition getters (sparse.hpp:527-552), with the invariants of
+    // tests/src/Matrix/sparse.cpp:56-89: boundaries run from 0 to the dimension,
+    // each thread's primary range holds its share of the non-zeros, and the
+    // per-primary non-zero sub-ranges cover exactly the entries whose secondary
+    // index lies in the thread's secondary range.
+    {
+        const auto& pb = sparse.get_primary_boundaries();
+        const auto& sb = sparse.get_secondary_boundaries();
+        const auto& nzb = sparse.get_secondary_nonzero_boundaries();
+        bool pok = pb.size() == num_threads + 1u && sb.size() == num_threads + 1u && nzb.size() == num_threads + 1u;
+        pok = pok && pb.front() == 0 && pb.back() == nc && sb.front() == 0 && sb.back() == nr;
+        const size_t total = ptr[nc];
+        for (int t = 0; pok && t < num_threads; ++t) {
+            pok = pb[t] <= pb[t + 1] && sb[t] <= sb[t + 1];
+            // A primary boundary sits where the running count first reaches the
+            // thread's cumulative share (floor(total/T), the first total%T one more).
+            size_t share = 0;
+            for (int u = 0; u <= t; ++u) share += total / num_threads + (static_cast<size_t>(u) < total % num_threads);
+            pok = pok && ptr[pb[t + 1]] <= share;
+            for (irlba::Index c = 0; pok && c < nc; ++c) {
+                for (size_t e = nzb[t][c]; e < nzb[t + 1][c]; ++e) pok = pok && idx[e] >= sb[t] && idx[e] < sb[t + 1];
+            }
+        }
+        for (irlba::Index c = 0; pok && c < nc; ++c) pok = nzb[0][c] == ptr[c] && nzb[num_threads][c] == ptr[c + 1];
+        check(pok, "HipSparseMatrix partition getters follow the reference's nnz-balanced split");
+        irlba::HipSparseMatrix<irlba::VectorXd, irlba::MatrixXd, std::vector<double>, std::vector<int>,
+                               std::vector<size_t> > single(nr, nc, x, idx, ptr, true, 1);
+        check(single.get_primary_boundaries().empty() && single.get_secondary_nonzero_boundaries().empty(),
+              "HipSparseMatrix with one thread leaves the partition empty");
+    }
+
+    // Integer values (e.g. counts) and 64-bit indices are converted on the host.
+    {
+        std::vector<int> xi;
+        std::vector<long> idx64(idx.begin(), idx.end());
+        for (size_t e = 0; e < x.size(); ++e) xi.push_back(static_cast<int>(e % 7));
+        irlba::MatrixXd Si(nr, nc);
+        for (irlba::Index c = 0; c < nc; ++c)
+            for (size_t e = ptr[c]; e < ptr[c + 1]; ++e) Si(idx[e], c) = xi[e];
+        irlba::HipSparseMatrix<irlba::VectorXd, irlba::MatrixXd, std::vector<int>, std::vector<long>,
+                               std::vector<size_t> > isparse(nr, nc, xi, idx64, ptr, true, 2);
+        irlba::VectorXd vv(nc), y1(nr), y2(nr);
+        for (irlba::Index i = 0; i < nc; ++i) vv[i] = std::cos(0.7 * i);
+        isparse.new_workspace()->multiply(vv, y1);
+        for (irlba::Index r = 0; r < nr; ++r) {
+            double acc = 0;
+            for (irlba::Index c = 0; c < nc; ++c) acc += Si(r, c) * vv[c];
+            y2[r] = acc;
+        }
+        bool iok = true;
+        for (irlba::Index r = 0; r < nr; ++r) iok = iok && std::abs(y1[r] - y2[r]) <= 1e-12 * (1 + std::abs(y2[r]));
+        check(iok, "HipSparseMatrix over integer values and 64-bit indices");
+        std::string emsg;
+        std::vector<long> bad(idx64);
+        bad[0] = static_cast<long>(nr) + (1L << 32);  // would wrap into range as int32
+        try {
+            irlba::HipSparseMatrix<irlba::VectorXd, irlba::MatrixXd, std::vector<int>, std::vector<long>,
+                                   std::vector<size_t> > b(nr, nc, xi, bad, ptr, true, 1);
+        } catch (const std::exception& e) {
+            emsg = e.what();
+        }
+        check(emsg.find("within the secondary dimension") != std::string::npos,
+              "HipSparseMatrix rejects a 64-bit index beyond the matrix");
+    }
 
     // Workspaces (interface.hpp): products of decorators over device and host matrices agree.
     irlba::VectorXd c(nc), s(nc), v(nc), o1(nr), o2(nr), u(nr), z1(nc), z2(nc);
@@ -188,6 +295,29 @@ int main() {
         msg = e.what();
     }
     check(msg.find("cannot scale") != std::string::npos, "error: cannot scale");
+
+    // A caller-defined Matrix whose product throws: compute() ends with the
+    // caller's own exception, as the reference's direct workspace calls give
+    // (lanczos.hpp:74,93,115).
+    {
+        std::string emsg;
+        bool right_type = false;
+        ThrowingDense thrower(A, 3);
+        try {
+            irlba::compute(thrower, 4, opt);
+        } catch (const UserError& e) {
+            right_type = true;
+            emsg = e.what();
+        } catch (const std::exception& e) {
+            emsg = e.what();
+        }
+        check(right_type && emsg == "user product failed after 3 calls",
+              "compute: a caller-defined product's exception reaches the caller intact");
+        auto again = irlba::compute(host, 6, opt);
+        bool aok = again.D.size() == 6;
+        for (irlba::Index i = 0; aok && i < 6; ++i) aok = same_same(again.D[i], r2.D[i], 1e-12);
+        check(aok, "compute: the context works after a caller exception");
+    }
 
     std::printf("%d failure(s)\n", failures);
     return failures == 0 ? 0 : 1;

@@ -105,3 +105,106 @@ def test_torch_binding_registers_operators():
     if not torch.cuda.is_available():
         with pytest.raises(RuntimeError):
             irlba.svd_dense(torch.randn(6, 4, dtype=torch.float64, requires_grad=True), 2)
+
+
+def test_negative_number_is_rejected_before_any_work():
+    """compute()/pca() with a negative count fail with a message (the reference
+    sizes its outputs from `number`, compute.hpp:141-160); checked before the
+    matrix is touched, so no device is needed."""
+    from cppirlba_amd._lib import lib
+
+    L = lib()
+    i64, i32 = ctypes.c_int64(), ctypes.c_int32()
+    d = ctypes.c_double()
+    assert L.irlba_hip_compute(None, -1, None, None, None, None, ctypes.byref(i64), ctypes.byref(i32),
+                               ctypes.byref(i32), 0) != 0
+    assert "non-negative" in L.irlba_hip_last_error().decode()
+    assert L.irlba_hip_pca(None, 1, 1, -3, None, None, None, None, ctypes.byref(i64), ctypes.byref(i32),
+                           ctypes.byref(i32), 0) != 0
+    assert "non-negative" in L.irlba_hip_last_error().decode()
+    assert d.value == 0
+
+
+def test_sparse_inputs_are_validated_on_the_host():
+    """HipSparseMatrix takes float64/float32 values, int32 indices and int64
+    pointers; other types are converted when that is exact and rejected
+    otherwise, before any device call."""
+    import numpy as np
+
+    import cppirlba_amd as irlba
+
+    ptr = np.array([0, 2, 3])
+    idx = np.array([0, 3, 1])
+    vals = np.array([1.0, 2.0, 3.0])
+    with pytest.raises(TypeError, match="real numbers"):
+        irlba.HipSparseMatrix(4, 2, vals.astype(np.complex128), idx, ptr)
+    with pytest.raises(TypeError, match="real numbers"):
+        irlba.HipSparseMatrix(4, 2, vals > 0, idx, ptr)
+    with pytest.raises(TypeError, match="integers"):
+        irlba.HipSparseMatrix(4, 2, vals, idx.astype(np.float64), ptr)
+    with pytest.raises(TypeError, match="integers"):
+        irlba.HipSparseMatrix(4, 2, vals, idx, ptr.astype(np.float32))
+    with pytest.raises(ValueError, match="do not fit in int32"):
+        irlba.HipSparseMatrix(4, 2, vals, np.array([0, 2 ** 32 + 1, 1]), ptr)
+    with pytest.raises(ValueError, match="one entry per column"):
+        irlba.HipSparseMatrix(4, 3, vals, idx, ptr)
+    with pytest.raises(ValueError, match="same length"):
+        irlba.HipSparseMatrix(4, 2, vals[:2], idx, ptr)
+    with pytest.raises(ValueError, match="num_threads"):
+        irlba.HipSparseMatrix(4, 2, vals, idx, ptr, True, 0)
+    # Exact conversions.
+    assert irlba._int_array(idx, "int32", "indices").dtype == np.int32
+    assert irlba._int_array(idx.astype(np.int32), "int64", "pointers").dtype == np.int64
+    v, t = irlba._sparse_values(vals.astype(np.float16))
+    assert v.dtype == np.float32 and t == irlba.F32
+    v, t = irlba._sparse_values(np.array([1, 2, 3], dtype=np.int16))
+    assert v.dtype == np.float64 and t == irlba.F64 and v.tolist() == [1.0, 2.0, 3.0]
+
+
+def test_torch_inputs_are_validated_on_the_host():
+    torch = pytest.importorskip("torch")
+    import cppirlba_amd as irlba
+
+    i64 = torch.tensor([0, 3, 1])
+    out = irlba._int_array(i64, "int32", "indices")
+    assert out.dtype == torch.int32 and out.tolist() == [0, 3, 1]
+    out = irlba._int_array(torch.tensor([0, 2, 3], dtype=torch.int32), "int64", "pointers")
+    assert out.dtype == torch.int64
+    with pytest.raises(ValueError, match="do not fit in int32"):
+        irlba._int_array(torch.tensor([0, 2 ** 31]), "int32", "indices")
+    with pytest.raises(TypeError, match="integers"):
+        irlba._int_array(torch.tensor([0.0, 1.0]), "int32", "indices")
+    v, t = irlba._sparse_values(torch.tensor([1.0, 2.5], dtype=torch.bfloat16))
+    assert v.dtype == torch.float32 and t == irlba.F32 and v.tolist() == [1.0, 2.5]
+    v, t = irlba._sparse_values(torch.tensor([1, 7], dtype=torch.int32))
+    assert v.dtype == torch.float64 and t == irlba.F64
+    with pytest.raises(TypeError, match="real numbers"):
+        irlba._sparse_values(torch.tensor([1 + 1j]))
+    with pytest.raises(TypeError, match="real numbers"):
+        irlba.HipDenseMatrix(torch.ones(3, 2, dtype=torch.bool))
+
+
+def test_dense_inputs_are_validated_on_the_host():
+    import numpy as np
+
+    import cppirlba_amd as irlba
+
+    with pytest.raises(TypeError, match="real numbers"):
+        irlba.HipDenseMatrix(np.ones((3, 2), dtype=np.complex64))
+    with pytest.raises(TypeError, match="real numbers"):
+        irlba.HipDenseMatrix(np.ones((3, 2), dtype=bool))
+
+
+def test_sddmm_pattern_is_checked_on_the_host():
+    """The SDDMM writes one value per stored index: pointers that do not end at
+    the index count (or do not start at 0, or decrease) are rejected before the
+    kernel could read past the indices or write past the output."""
+    import numpy as np
+
+    import cppirlba_amd as irlba
+
+    X, Y = np.ones((4, 2)), np.ones((3, 2))
+    idx = np.array([0, 1, 2, 3])
+    for ptr in ([0, 2, 3, 5], [1, 2, 3, 4], [0, 3, 2, 4], [0, 1, 2, 3]):
+        with pytest.raises(ValueError, match="pointers"):
+            irlba.sddmm(4, 3, idx, np.array(ptr), X, Y)
