@@ -62,7 +62,7 @@ def cpu_baseline(nr, ncol_full, density, seed, nnz_full, n_fwd, n_adj, restarts,
 
     from oracle import oracle as O
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = max(1, len(os.sched_getaffinity(0)))  # every core this process may run on
     vals, idx, ptr = O.gen_nb_sparse(nr, sample_cols, density, seed, line0=0, line1=sample_cols, f32=True)
     A = O.sparse(nr, sample_cols, vals, idx, ptr, True, threads)
     x = np.random.default_rng(0).standard_normal(sample_cols)
@@ -93,7 +93,8 @@ def cpu_baseline(nr, ncol_full, density, seed, nnz_full, n_fwd, n_adj, restarts,
         "unit": "s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"reference algorithm restated (no Eigen), {threads} threads for the products: a "
+        "sample": (f"reference algorithm restated (no Eigen), {threads} threads (the cores this process may "
+                   f"use) for the products: a "
                    f"{nr}x{sample_cols} column slice of the same generated matrix ({int(ptr[-1])} nnz); "
                    f"value = A.v {fwd * 1e3:.1f} ms + A'.u {adj * 1e3:.1f} ms per full-size product (median of 5, "
                    f"scaled by nnz) x the GPU solve's {n_fwd} + {n_adj} products (products only, a lower bound). "
@@ -221,15 +222,30 @@ def backward_leg(args):
     res["sparse_nnz"] = int(idx.numel())
     GU = torch.randn(m, k, generator=gen, device=dev, dtype=torch.float64)
     GV = torch.randn(n, k, generator=gen, device=dev, dtype=torch.float64)
-    comps = ("rank_k", "exact") if args.backward_exact_sparse else ("rank_k",)
+    comps = ("rank_k",) if args.skip_backward_exact_sparse else ("rank_k", "exact")
     for comp in comps:
         t0 = time.perf_counter()
-        _, times = _time_backward(torch, lambda: irlba.svd_sparse(vals, idx, ptr, m, n, k, options=opts,
-                                                                  storage="f32", complement=comp),
-                                  loss_fn(w, GU, GV), args.reps)
+        out, times = _time_backward(torch, lambda: irlba.svd_sparse(vals, idx, ptr, m, n, k, options=opts,
+                                                                    storage="f32", complement=comp),
+                                    loss_fn(w, GU, GV), args.reps)
         res[f"sparse_{m}x{n}_{comp}_ms"] = round(float(np.median(times)), 3)
         log(f"  backward sparse {m}x{n} nnz={idx.numel()} ({comp}): {np.median(times):.3f} ms "
             f"(forward + {args.reps + 1} backward {time.perf_counter() - t0:.2f} s)")
+        if comp == "exact":
+            # The CG iteration count of the exact complement (the same backward
+            # through the C ABI on a handle of the same values), and the gap that
+            # sets it: the projected systems are shifted by s_16^2 against the
+            # spectrum below, so they converge at a rate set by s_16 - s_17.
+            U, S, V = (t.detach() for t in out)
+            h = irlba.HipSparseMatrix(m, n, vals.detach(), idx, ptr, storage="f32")
+            bw = irlba.svd_backward(h, U.cpu().numpy(), S.cpu().numpy(), V.cpu().numpy(), w.cpu().numpy(),
+                                    GU.cpu().numpy(), GV.cpu().numpy(), complement="exact")
+            s17 = irlba.compute(h, k + 1, opts).D
+            res[f"sparse_{m}x{n}_exact_cg_iterations"] = int(bw.cg_iterations)
+            res[f"sparse_{m}x{n}_s16_s17"] = [round(float(s17[k - 1]), 6), round(float(s17[k]), 6)]
+            log(f"    exact complement: {bw.cg_iterations} lock-step CG iterations; s_16 {s17[k - 1]:.5f}, "
+                f"s_17 {s17[k]:.5f}")
+            del h
     return res
 
 
@@ -323,8 +339,8 @@ def main():
     ap.add_argument("--dense-cols", type=int, default=2000)
     ap.add_argument("--dense-reps", type=int, default=1)
     ap.add_argument("--dense-storages", default="bf16,f32,f64")
-    ap.add_argument("--backward-exact-sparse", action="store_true",
-                    help="also time the exact (CG) complement on the sparse backward case")
+    ap.add_argument("--skip-backward-exact-sparse", action="store_true",
+                    help="time only the rank-k complement on the sparse backward case")
     ap.add_argument("--cpu-sample-cols", type=int, default=33334)
     args = ap.parse_args()
 
@@ -396,7 +412,16 @@ def main():
     dominant = "adjoint" if adj_ms * n_adj >= fwd_ms * n_fwd else "forward"
     dom_ms = adj_ms if dominant == "adjoint" else fwd_ms
     dom_bytes = st["adjoint_bytes"] if dominant == "adjoint" else st["forward_bytes"]
-    achieved = dom_bytes / (dom_ms * 1e6)
+    # The bytes one launch of the dominant product must move in the layout it
+    # streams: every stored slot (value + 1-byte delta index; entries, padding
+    # and gap-bridging slots), the segment pointers and first indices, and the
+    # FP64 input and output vectors.
+    lay = mixed.layout(dominant == "adjoint")
+    n_major = nc if dominant == "adjoint" else nr
+    nseg = lay["n_tiles"] * n_major
+    stored_bytes = int(lay["slots"] * 4 + lay["index_bytes"] + (nseg + 1) * 8 + (nseg * 2 if lay["delta"] else 0)
+                       + (nr + nc) * 8)
+    achieved = stored_bytes / (dom_ms * 1e6)
 
     out = {
         "metric": METRIC,
@@ -448,27 +473,28 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "note": "achieved counts the int32-index format bytes (nnz x 8 + pointers + vectors); the stored layout "
-                    "(8-bit delta indices, 5 bytes per slot) moves fewer, so frac can exceed 1; traffic_frac "
-                    "(measured HBM bytes per launch / launch time / peak) is the HBM utilisation",
+            "bytes_per_launch": stored_bytes,
+            "launch_ms": round(dom_ms, 4),
+            "note": "achieved = bytes of the layout one launch streams (slots x (4-byte FP32 value + 1-byte delta "
+                    "index) + segment pointers and first indices + FP64 input/output vectors) / mean launch time "
+                    "(HIP events on the engine's stream); frac_int32_format_equiv divides the nnz x (4 + 4-byte "
+                    "int32 index) format bytes of the workload definition instead, which no launch reads",
+            "frac_int32_format_equiv": round(dom_bytes / (dom_ms * 1e6) / HBM_PEAK_GBS, 4),
+            "layout": lay,
         },
     }
-    lay = mixed.layout(dominant == "adjoint")
-    n_major = nc if dominant == "adjoint" else nr
-    nseg = lay["n_tiles"] * n_major
-    out["roofline"]["stored_bytes"] = int(lay["slots"] * 4 + lay["index_bytes"] + (nseg + 1) * 8
-                                          + (nseg * 2 if lay["delta"] else 0) + (nr + nc) * 8)
-    out["roofline"]["layout"] = lay
     prof = os.path.join(ROOT, "profiles", "spmv_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:
             tr = json.load(f)
         key = f"{dominant}_hbm_bytes_per_launch"
         if key in tr:
+            # Measured HBM bytes of one launch (calibrated FETCH_SIZE + WRITE_SIZE) from
+            # the committed counter profile of this workload, over this run's launch time.
             out["roofline"]["traffic"] = tr[key]
-            # Measured HBM bytes over the same launch time: the stored layout keeps
-            # 1-byte delta indices, so it moves less than the algorithmic figure.
-            out["roofline"]["traffic_GBps"] = round(tr[key] / (dom_ms * 1e6), 1)
+            out["roofline"]["traffic_source"] = ("profiles/spmv_traffic.json (rocprofv3 --pmc FETCH_SIZE / "
+                                                 "WRITE_SIZE runs of bench.py --mode spmv, calibrated by a copy "
+                                                 "kernel); not measured in this run")
             out["roofline"]["traffic_frac"] = round(tr[key] / (dom_ms * 1e6) / HBM_PEAK_GBS, 4)
 
     if not args.skip_fp64:

@@ -71,6 +71,36 @@ def test_lanczos_sweep_vs_oracle(nr, nc, work, start):
     np.testing.assert_allclose(hV.T @ hV, np.eye(work), atol=1e-11)
 
 
+@pytest.mark.parametrize("rank", [1, 2, 3, 5])
+def test_lanczos_sweep_refills_vs_oracle(rank):
+    # An exactly rank-deficient matrix: after `rank` steps the residual norms fall
+    # to rounding level and every later step refills F and W_next with normals from
+    # std::mt19937_64(seed) (lanczos.hpp:99-109,123-132).  The refilled columns of
+    # V and W are those draws orthogonalised, so they pin the engine's draw order
+    # and values to the oracle's.  The refill threshold eps^0.8 ~ 3e-13 is absolute
+    # (lanczos.hpp:65-66), so the matrix is scaled to unit norm: its rounding-level
+    # residuals (~1e-15) then fall clearly below it in both summation orders, where
+    # at norm ~50 they sit at ~3e-13 and which side they land on is rounding.
+    nr, nc, work = 60, 40, 9
+    base = O.create_random_matrix(nr, rank, 4)
+    A = np.empty((nr, nc))
+    for i in range(nc):
+        A[:, i] = base[:, i % rank]
+    A /= np.linalg.norm(A, 2)
+    V = np.zeros((nc, work))
+    W = np.zeros((nr, work))
+    B = np.zeros((work, work))
+    v0 = np.random.default_rng(rank).standard_normal(nc)
+    V[:, 0] = v0 / np.linalg.norm(v0)
+    oW, oV, oB, oF = O.lanczos(O.dense(A), W, V, B, 123, 0)
+    hW, hV, hB, hF = irlba.run_lanczos_bidiagonalization(irlba.HipDenseMatrix(A), W, V, B, 123, 0)
+    assert np.sum(np.abs(np.diag(oB, 1)) < 1e-12) >= 1  # refills happened (R_F = 0)
+    np.testing.assert_allclose(hB, oB, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(hV, oV, rtol=1e-8, atol=1e-10)
+    np.testing.assert_allclose(hW, oW, rtol=1e-8, atol=1e-10)
+    np.testing.assert_allclose(hV.T @ hV, np.eye(work), atol=1e-11)
+
+
 def test_lanczos_inputs_land_after_the_zero_fill():
     # The standalone sweep zero-fills its new bases on the context's stream and
     # then uploads the caller's W and V; both go through that stream, so the
