@@ -121,10 +121,11 @@ def test_sparse_pca_30000x1e6_mixed_and_fp64():
     angle = subspace_angle(mixed.rotation, f64.rotation[:, :k])
     _say(f"  Wedin bounds held for {checked} of {k} leading subspaces; 50-dim rotation subspace angle "
          f"mixed vs fp64 {angle:.3e} rad")
-    assert checked >= 1
-    # Where no bound applies (the 50-dim subspace itself) the angle is what the
-    # tolerance leaves: measured 2.7e-4 to 1.1e-3 rad across runs; asserted below 1e-2.
-    assert angle < 1e-2
+    # On this matrix the tolerance resolves 49 of the 50 leading subspaces (all
+    # but the 50-dim one, whose last vector mixes with the 51st: its angle is
+    # reported, not bounded; it depends on the run, e.g. 1.1e-3 rad against an
+    # FP64 top-50 run and 1.5e-2 against this top-51 one).
+    assert checked >= 45
 
 
 # ---------------------------------------------------------------- dense 200000 x 2000
