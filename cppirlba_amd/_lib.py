@@ -141,6 +141,7 @@ SIGNATURES.update({
     "irlba_hip_svd_backward": (_i, [_VP, _i64, _VP, _VP, _VP, _VP, _VP, _VP, _i, ctypes.c_int32, ctypes.c_double,
                                     _VP, _VP, _VP, _I32, _i]),
     "irlba_hip_sddmm": (_i, [_VP, _i64, _i64, _VP, _VP, _i, _i64, _VP, _VP, _VP, _i, _i]),
+    "irlba_hip_sddmm_noalloc": (_i, [_VP, _i64, _i64, _VP, _VP, _i, _i64, _VP, _VP, _VP, _i]),
     "irlba_hip_multiply_block": (_i, [_VP, _i, _i64, _VP, _VP, _i]),
     "irlba_hip_sparse_layout": (_i, [_VP, _i, ctypes.POINTER(Layout)]),
 })

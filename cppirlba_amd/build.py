@@ -31,6 +31,7 @@ SOURCES = [
     "kernels_generate.hip",
     "kernels_backward.hip",
     "kernels_spmm.hip",
+    "noalloc.hip",
 ]
 
 # Host-only C++ (no device code), compiled by g++.

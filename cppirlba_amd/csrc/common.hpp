@@ -310,6 +310,13 @@ i64 dense_block_scratch(const DenseDesc& d, int kp);
 void launch_dense_block(const DenseDesc& d, bool adjoint, int kp, bool x_f32, const void* X, double* Y,
                         double* scratch, i64 scratch_cap, hipStream_t s);
 
+// Fixed device scratch of the allocation-free entry point (noalloc.hip): its
+// size, its device address on `device` (resolved once per device, with that
+// device current) and a kernel that clears an int flag in it.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+void* noalloc_scratch(int device);
+void launch_clear_flag(int* flag, hipStream_t s);
+
 // Synthetic data (counter-based, identical to the oracle's generator).
 void launch_gen_nb_count(i64 n_minor, i64 n_major, std::uint32_t thr, std::uint64_t seed, i64* counts,
                          hipStream_t s);

@@ -151,6 +151,9 @@ struct irlba_hip_context {
         if (!copy_stream) IRLBA_HIP_CHECK(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
         if (!copy_done) IRLBA_HIP_CHECK(hipEventCreateWithFlags(&copy_done, hipEventDisableTiming));
     }
+    // Device address of the fixed scratch of the allocation-free entry point on
+    // this context's device (resolved when the context is created).
+    void* noalloc_scratch = nullptr;
     // A device int cleared in stream order, for the error bits of checking kernels.
     std::shared_ptr<int> flag_buf;
     int* flag() {
@@ -1551,6 +1554,7 @@ int irlba_hip_context_create(int device, irlba_hip_context** out) {
         ctx->cu_count = prop.multiProcessorCount;
         ctx->lds_per_block = prop.sharedMemPerBlock;
         ctx->lds_per_cu = std::max<size_t>(prop.maxSharedMemoryPerMultiProcessor, prop.sharedMemPerBlock);
+        ctx->noalloc_scratch = noalloc_scratch(device);
         *out = ctx.release();
     });
 }
@@ -2156,6 +2160,46 @@ int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const in
             IRLBA_HIP_CHECK(hipMemcpyAsync(out, dout, ob * nnz, hipMemcpyDeviceToHost, s));
             IRLBA_HIP_CHECK(hipStreamSynchronize(s));
         }
+    });
+}
+
+int irlba_hip_sddmm_noalloc(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_t* indices,
+                            const int64_t* pointers, int column_major, int64_t rank, const double* X, const double* Y,
+                            void* out, int out_type) {
+    return guarded([&] {
+        require(ctx && indices && pointers && X && Y && out, "null argument");
+        require(nrow >= 0 && ncol >= 0, "negative dimension");
+        require(rank >= 1 && rank <= 128, "rank must lie in [1, 128]");
+        require(out_type == IRLBA_HIP_F64 || out_type == IRLBA_HIP_F32, "out_type must be F64 or F32");
+        const bool f32 = out_type == IRLBA_HIP_F32;
+        const size_t ob = f32 ? 4 : 8;
+        const int Rp = sddmm_padded_rank(static_cast<int>(rank), f32);
+        // The error flag in the first 256 bytes, then both factors packed
+        // row-major (stride Rp) in the output precision, as irlba_hip_sddmm
+        // packs them in its workspace.
+        const size_t need = 256 + ob * static_cast<size_t>(Rp) * static_cast<size_t>(nrow + ncol);
+        if (need > kNoAllocScratchBytes)
+            throw std::runtime_error("irlba_hip_sddmm_noalloc: the packed factors need " + std::to_string(need) +
+                                     " bytes of scratch, above its fixed " + std::to_string(kNoAllocScratchBytes) +
+                                     " bytes ((nrow + ncol) x padded rank x value size + 256); use irlba_hip_sddmm");
+        CtxGuard g(ctx);
+        require(ctx->noalloc_scratch != nullptr, "the fixed scratch is not available on this device");
+        hipStream_t s = ctx->stream;
+        char* base = static_cast<char*>(ctx->noalloc_scratch);
+        int* flag = reinterpret_cast<int*>(base);
+        char* pX = base + 256;
+        char* pY = pX + ob * static_cast<size_t>(Rp) * static_cast<size_t>(nrow);
+        const i64 n_lines = column_major ? ncol : nrow, n_other = column_major ? nrow : ncol;
+        launch_clear_flag(flag, s);
+        launch_pack_rows(X, nrow, nrow, static_cast<int>(rank), Rp, f32, pX, s);
+        launch_pack_rows(Y, ncol, ncol, static_cast<int>(rank), Rp, f32, pY, s);
+        launch_sddmm(indices, pointers, n_lines, n_other, column_major ? pY : pX, column_major ? pX : pY,
+                     static_cast<int>(rank), f32, out, flag, s);
+        int hflag = 0;
+        IRLBA_HIP_CHECK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+        require(!(hflag & 1), "pointers are not a valid compressed layout");
+        require(!(hflag & 2), "indices out of range");
     });
 }
 

@@ -247,6 +247,22 @@ int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const in
                     const int64_t* pointers, int column_major, int64_t rank, const double* X, const double* Y,
                     void* out, int out_type, int device_pointers);
 
+/* irlba_hip_sddmm without any device allocation.  No reference counterpart:
+ * it exists for callers that cannot allocate device memory (code under stream
+ * capture, tight training or serving loops).  Same arguments and results as
+ * irlba_hip_sddmm with device_pointers = 1 (every array on the context's
+ * device, none NULL); its temporaries (the error flag and both factors packed
+ * row-major) live in a fixed per-device scratch array of 32 MiB inside the
+ * library, held by every GPU the library runs on until the process ends.  A
+ * call whose packed factors need more, 256 + (nrow + ncol) x padded rank x
+ * value size bytes (padded rank: rank rounded up to a power-of-two multiple of
+ * 2 for F64 and of 4 for F32 output), fails with an error and writes nothing.
+ * All calls on one GPU share the array: calls that overlap in time on the same
+ * device (through two contexts, or two threads) race on it. */
+int irlba_hip_sddmm_noalloc(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_t* indices,
+                            const int64_t* pointers, int column_major, int64_t rank, const double* X, const double* Y,
+                            void* out, int out_type);
+
 #ifdef __cplusplus
 }
 #endif
