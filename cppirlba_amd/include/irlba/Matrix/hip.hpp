@@ -163,6 +163,60 @@ const T* as_contiguous(const Array_& a, std::size_t n, std::vector<T>& tmp) {
     }
 }
 
+template <class PointerType_>
+struct SparsePartition {
+    std::vector<Index> primary, secondary;
+    std::vector<std::vector<PointerType_> > nonzero;
+};
+
+// The thread partition of ParallelSparseMatrix (sparse.hpp:52-114), for the
+// getters of HipSparseMatrix: thread t gets an equal share of the non-zeros
+// (the first total % T threads one more), as a range of primary elements and
+// as a range of secondary elements, plus, per primary element, the sub-range
+// of its non-zeros whose secondary index falls in each thread's secondary
+// range.  Empty when num_threads <= 1, where the reference leaves it unset.
+template <class PointerType_, class PointerArray_, class IndexArray_>
+SparsePartition<PointerType_> sparse_partition(Index n_primary, Index n_secondary, const PointerArray_& ptrs,
+                                               const IndexArray_& indices, int num_threads) {
+    SparsePartition<PointerType_> part;
+    const int T = num_threads;
+    if (T <= 1) return part;
+    const PointerType_ total = ptrs[n_primary];
+    const PointerType_ floor_ = total / T;
+    const PointerType_ extra = total % T;
+    part.primary.assign(static_cast<std::size_t>(T) + 1, 0);
+    Index pc = 0;
+    PointerType_ sofar = 0;
+    for (int t = 0; t < T; ++t) {
+        sofar += floor_ + (static_cast<PointerType_>(t) < extra ? 1 : 0);
+        while (pc < n_primary && ptrs[pc + 1] <= sofar) ++pc;
+        part.primary[t + 1] = pc;
+    }
+    std::vector<PointerType_> per(static_cast<std::size_t>(n_secondary), 0);
+    for (PointerType_ e = 0; e < total; ++e) ++per[static_cast<std::size_t>(indices[e])];
+    part.secondary.assign(static_cast<std::size_t>(T) + 1, 0);
+    Index sc = 0;
+    PointerType_ cum = 0;
+    sofar = 0;
+    for (int t = 0; t < T; ++t) {
+        sofar += floor_ + (static_cast<PointerType_>(t) < extra ? 1 : 0);
+        while (sc < n_secondary && cum <= sofar) cum += per[static_cast<std::size_t>(sc++)];
+        part.secondary[t + 1] = sc;
+    }
+    part.nonzero.assign(static_cast<std::size_t>(T) + 1, std::vector<PointerType_>(static_cast<std::size_t>(n_primary)));
+    for (Index c = 0; c < n_primary; ++c) {
+        const PointerType_ end = ptrs[c + 1];
+        PointerType_ e = ptrs[c];
+        part.nonzero[0][c] = e;
+        for (int t = 0; t < T; ++t) {
+            const Index lim = part.secondary[t + 1];
+            while (e < end && static_cast<Index>(indices[e]) < lim) ++e;
+            part.nonzero[t + 1][c] = e;
+        }
+    }
+    return part;
+}
+
 }  // namespace detail
 
 // Drop-in for ParallelSparseMatrix<EigenVector_, EigenMatrix_, ValueArray_,
@@ -276,59 +330,14 @@ private:
     int my_num_threads = 1;
     Index my_primary = 0, my_secondary = 0;
 
-    struct Partition {
-        std::vector<Index> primary, secondary;
-        std::vector<std::vector<PointerType> > nonzero;
-    };
+    typedef detail::SparsePartition<PointerType> Partition;
     mutable std::unique_ptr<Partition> my_part;
     mutable std::once_flag my_part_once;
 
-    // The partition of sparse.hpp:52-114: thread t gets an equal share of the
-    // non-zeros (the first total % T threads one more), as a range of primary
-    // elements and as a range of secondary elements, plus, per primary element,
-    // the sub-range of its non-zeros whose secondary index falls in each
-    // thread's secondary range.
     void partition() const {
         std::call_once(my_part_once, [this] {
-            auto part = std::make_unique<Partition>();
-            const int T = my_num_threads;
-            if (T > 1) {
-                const PointerType total = my_ptrs[my_primary];
-                const PointerType floor_ = total / T;
-                const PointerType extra = total % T;
-                part->primary.assign(static_cast<std::size_t>(T) + 1, 0);
-                Index pc = 0;
-                PointerType sofar = 0;
-                for (int t = 0; t < T; ++t) {
-                    sofar += floor_ + (static_cast<PointerType>(t) < extra ? 1 : 0);
-                    while (pc < my_primary && my_ptrs[pc + 1] <= sofar) ++pc;
-                    part->primary[t + 1] = pc;
-                }
-                std::vector<PointerType> per(static_cast<std::size_t>(my_secondary), 0);
-                for (PointerType e = 0; e < total; ++e) ++per[static_cast<std::size_t>(my_indices[e])];
-                part->secondary.assign(static_cast<std::size_t>(T) + 1, 0);
-                Index sc = 0;
-                PointerType cum = 0;
-                sofar = 0;
-                for (int t = 0; t < T; ++t) {
-                    sofar += floor_ + (static_cast<PointerType>(t) < extra ? 1 : 0);
-                    while (sc < my_secondary && cum <= sofar) cum += per[static_cast<std::size_t>(sc++)];
-                    part->secondary[t + 1] = sc;
-                }
-                part->nonzero.assign(static_cast<std::size_t>(T) + 1,
-                                     std::vector<PointerType>(static_cast<std::size_t>(my_primary)));
-                for (Index c = 0; c < my_primary; ++c) {
-                    const PointerType end = my_ptrs[c + 1];
-                    PointerType e = my_ptrs[c];
-                    part->nonzero[0][c] = e;
-                    for (int t = 0; t < T; ++t) {
-                        const Index lim = part->secondary[t + 1];
-                        while (e < end && static_cast<Index>(my_indices[e]) < lim) ++e;
-                        part->nonzero[t + 1][c] = e;
-                    }
-                }
-            }
-            my_part = std::move(part);
+            my_part = std::make_unique<Partition>(
+                detail::sparse_partition<PointerType>(my_primary, my_secondary, my_ptrs, my_indices, my_num_threads));
         });
     }
 };
