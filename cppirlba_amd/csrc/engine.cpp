@@ -71,6 +71,20 @@ std::shared_ptr<void> dalloc_bytes(size_t bytes) {
     return std::shared_ptr<void>(p, [](void* q) { (void)hipFree(q); });
 }
 
+// Pinned host memory (hipHostMalloc), freed with its owner.
+struct PinnedBuffer {
+    double* p = nullptr;
+    explicit PinnedBuffer(size_t bytes) {
+        IRLBA_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(bytes, 8)));
+    }
+    ~PinnedBuffer() {
+        if (p) (void)hipHostFree(p);
+    }
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    double* get() const { return p; }
+};
+
 size_t storage_bytes(StorageType st) { return st == ST_F64 ? 8 : (st == ST_F32 ? 4 : 2); }
 
 StorageType to_storage(int t) {
@@ -1308,8 +1322,7 @@ struct LockstepCg {
         // ap = basis (basis' ap) - ap + sig2 in
         launch_gram(c.basis, c.L, c.ap, c.L, k, c.L, c.partials, c.proj, ldy, s);
         launch_rotate(c.basis, c.L, k, c.proj, ldy, k, c.uc, c.L, c.L, s);
-        launch_cols_axpby(c.ap, c.L, nullptr, -1.0, c.uc, c.L, nullptr, 1.0, k, c.L, s);
-        launch_cols_axpby(c.ap, c.L, nullptr, 1.0, in, c.L, c.sig2, 1.0, k, c.L, s);
+        launch_cg_op_finish(c.ap, c.uc, in, c.sig2, c.L, k, s);
     }
     void start(CgSet& c) {
         // x0 = b / sig2 (the rank-k answer), r0 = b - op(x0), p0 = r0.
@@ -1324,12 +1337,11 @@ struct LockstepCg {
     void step(CgSet& c) {
         op(c, c.p);
         launch_coldots(c.p, c.L, c.ap, c.L, k, c.L, c.partials, c.pap, s);
-        launch_cg_coefs(c.rr, c.pap, c.alpha, k, 0, s);  // alpha = rr / pap (0 where pap <= 0)
-        launch_cols_axpby(c.x, c.L, nullptr, 1.0, c.p, c.L, c.alpha, 1.0, k, c.L, s);
-        launch_cols_axpby(c.r, c.L, nullptr, 1.0, c.ap, c.L, c.alpha, -1.0, k, c.L, s);
-        launch_coldots(c.r, c.L, c.r, c.L, k, c.L, c.partials, c.rr_new, s);
-        launch_cg_coefs(c.rr_new, c.rr, c.beta, k, 1, s);  // beta = rr_new / rr; rr = rr_new
-        launch_cols_axpby(c.p, c.L, c.beta, 1.0, c.r, c.L, nullptr, 1.0, k, c.L, s);
+        // alpha = rr / pap (0 where pap <= 0); x += alpha p; r -= alpha ap; rr_new = |r|^2
+        launch_cg_update_xr(c.x, c.r, c.p, c.ap, c.rr, c.pap, c.L, k, c.partials, c.rr_new, s);
+        // beta = rr_new / rr; p = beta p + r; rr_new becomes rr (the two buffers swap)
+        launch_cg_update_p(c.p, c.r, c.rr_new, c.rr, c.L, k, s);
+        std::swap(c.rr, c.rr_new);
     }
     void finish(CgSet& c) {
         // Drop the rounding drift of x back out of span(basis).
@@ -1467,26 +1479,45 @@ int run_svd_backward(const irlba_hip_matrix* m, i64 k, const double* U, const do
             launch_cols_axpby(c.b, c.L, nullptr, -1.0, c.left ? dGU : dGV, c.L, nullptr, 1.0, ki, c.L, s);
             cg.start(c);
         }
-        // Convergence: |r_i| <= tol |b_i| for every system of both sides.
-        std::vector<double> bb(2 * k), rr(2 * k);
+        // Convergence: |r_i| <= tol |b_i| for every system of both sides.  The
+        // residual norms of each iteration are copied to pinned host memory and
+        // checked while the next iteration already runs on the device, so the
+        // GPU does not idle for the check; the step queued past convergence only
+        // refines the solutions.
         double* bnorm = wp[at - 1] + 5 * k;  // spare slot of the right set's scalars (k entries)
         launch_coldots(set[0].b, nr, set[0].b, nr, ki, nr, partials, set[0].beta, s);
         launch_coldots(set[1].b, nc, set[1].b, nc, ki, nc, partials, bnorm, s);
-        IRLBA_HIP_CHECK(hipMemcpyAsync(bb.data(), set[0].beta, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-        IRLBA_HIP_CHECK(hipMemcpyAsync(bb.data() + k, bnorm, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-        auto converged = [&]() {
-            IRLBA_HIP_CHECK(hipMemcpyAsync(rr.data(), set[0].rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-            IRLBA_HIP_CHECK(hipMemcpyAsync(rr.data() + k, set[1].rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-            ctx->sync();  // also returns the products' timing events to the pool
+        PinnedBuffer host(sizeof(double) * 6 * k);  // |b|^2 (2k), then two slots of |r|^2 (2k each)
+        double* bb = host.get();
+        IRLBA_HIP_CHECK(hipMemcpyAsync(bb, set[0].beta, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        IRLBA_HIP_CHECK(hipMemcpyAsync(bb + k, bnorm, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        hipEvent_t ready[2] = {ctx->get_event(), ctx->get_event()};
+        auto queue_check = [&](int slot) {
+            double* rr = bb + 2 * k + slot * 2 * k;
+            IRLBA_HIP_CHECK(hipMemcpyAsync(rr, set[0].rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+            IRLBA_HIP_CHECK(hipMemcpyAsync(rr + k, set[1].rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+            IRLBA_HIP_CHECK(hipEventRecord(ready[slot], s));
+        };
+        auto converged = [&](int slot) {
+            IRLBA_HIP_CHECK(hipEventSynchronize(ready[slot]));
+            const double* rr = bb + 2 * k + slot * 2 * k;
             for (i64 i = 0; i < 2 * k; ++i)
                 if (!(rr[i] <= cg_tol * cg_tol * bb[i])) return false;
             return true;
         };
-        while (iterations < max_cg && !converged()) {
+        int slot = 0;
+        queue_check(slot);
+        while (iterations < max_cg) {
             cg.step(set[0]);
             cg.step(set[1]);
             ++iterations;
+            queue_check(slot ^ 1);
+            if (converged(slot)) break;
+            slot ^= 1;
         }
+        IRLBA_HIP_CHECK(hipStreamSynchronize(s));  // every check copy has landed before the buffer goes
+        ctx->pool.push_back(ready[0]);
+        ctx->pool.push_back(ready[1]);
         cg.finish(set[0]);
         cg.finish(set[1]);
         // Z' = Z S + A T into X1;  Y' = A' Z + T S into Y0.  The right set's

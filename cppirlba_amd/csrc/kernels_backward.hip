@@ -358,6 +358,86 @@ void launch_cg_coefs(const double* a, double* b, double* out, int k, int mode, h
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
+// Fused steps of the lock-step CG (one kernel where four or two were), with
+// the arithmetic of the unfused forms, so the results are the same bit for bit.
+//   k_cg_op_finish: ap[:, i] = (basis (basis' ap))[:, i] - ap[:, i] + sig2[i] in[:, i]
+//   k_cg_update_xr: alpha_i = rr_i / pap_i (0 unless pap_i > 0); x += alpha p;
+//                   r -= alpha ap; per-block partial sums of r_i^2 in the layout
+//                   of k_coldots (grid gram_blocks(L) x k), reduced by k_coldots_reduce
+//   k_cg_update_p:  beta_i = rr_new_i / rr_i (0 unless rr_i > 0); p = p beta + r
+__global__ void k_cg_op_finish(double* __restrict__ ap, const double* __restrict__ uc, const double* __restrict__ in,
+                               const double* __restrict__ sig2, i64 L, int k) {
+    const i64 n = L * k;
+    for (i64 e = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<i64>(gridDim.x) * blockDim.x) {
+        const int i = static_cast<int>(e / L);
+        const double t = ap[e] * -1.0 + uc[e] * 1.0;
+        ap[e] = t * 1.0 + in[e] * (sig2[i] * 1.0);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cg_update_xr(double* __restrict__ x, double* __restrict__ r,
+                                                      const double* __restrict__ p, const double* __restrict__ ap,
+                                                      const double* __restrict__ rr, const double* __restrict__ pap,
+                                                      i64 L, i64 chunk, double* __restrict__ partials) {
+    __shared__ double red[8];
+    const int i = blockIdx.y;
+    const double pv = pap[i];
+    const double alpha = pv > 0 ? rr[i] / pv : 0.0;
+    const i64 r0 = static_cast<i64>(blockIdx.x) * chunk, r1 = std::min<i64>(L, r0 + chunk);
+    const i64 off = static_cast<i64>(i) * L;
+    double s = 0, unused = 0;
+    for (i64 l = r0 + threadIdx.x; l < r1; l += blockDim.x) {
+        x[off + l] = x[off + l] * 1.0 + p[off + l] * (alpha * 1.0);
+        const double rv = r[off + l] * 1.0 + ap[off + l] * (alpha * -1.0);
+        r[off + l] = rv;
+        s += rv * rv;
+    }
+    block_sum2(s, unused, red);
+    if (threadIdx.x == 0) partials[static_cast<i64>(i) * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void k_cg_update_p(double* __restrict__ p, const double* __restrict__ r, const double* __restrict__ rr_new,
+                              const double* __restrict__ rr, i64 L, int k) {
+    const i64 n = L * k;
+    for (i64 e = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<i64>(gridDim.x) * blockDim.x) {
+        const int i = static_cast<int>(e / L);
+        const double rv = rr[i];
+        const double beta = rv > 0 ? rr_new[i] / rv : 0.0;
+        p[e] = p[e] * (beta * 1.0) + r[e] * 1.0;
+    }
+}
+
+static i64 cols_grid(i64 L, int k) { return std::max<i64>(1, std::min<i64>((L * k + 255) / 256, 8192)); }
+
+void launch_cg_op_finish(double* ap, const double* uc, const double* in, const double* sig2, i64 L, int k,
+                         hipStream_t s) {
+    if (L <= 0 || k <= 0) return;
+    hipLaunchKernelGGL(k_cg_op_finish, dim3(static_cast<unsigned>(cols_grid(L, k))), dim3(256), 0, s, ap, uc, in, sig2,
+                       L, k);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+void launch_cg_update_xr(double* x, double* r, const double* p, const double* ap, const double* rr, const double* pap,
+                         i64 L, int k, double* partials, double* rr_new, hipStream_t s) {
+    if (k <= 0) return;
+    const int nb = gram_blocks(L);
+    const i64 chunk = (L + nb - 1) / nb;
+    hipLaunchKernelGGL(k_cg_update_xr, dim3(nb, k), dim3(256), 0, s, x, r, p, ap, rr, pap, L, chunk, partials);
+    IRLBA_HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_coldots_reduce, dim3(k), dim3(256), 0, s, partials, nb, k, rr_new);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+void launch_cg_update_p(double* p, const double* r, const double* rr_new, const double* rr, i64 L, int k,
+                        hipStream_t s) {
+    if (L <= 0 || k <= 0) return;
+    hipLaunchKernelGGL(k_cg_update_p, dim3(static_cast<unsigned>(cols_grid(L, k))), dim3(256), 0, s, p, r, rr_new, rr,
+                       L, k);
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
 // out[i] = a[i] * alpha + b[i] * beta  (beta ignored when b is null); the CG and
 // the adjoint assembly use it for their vector updates.
 __global__ void k_axpby(double* __restrict__ out, const double* __restrict__ a, double alpha,
