@@ -49,12 +49,12 @@ def test_sparse_pca_30000x1e6_mixed_and_fp64():
     nr, nc, k, tol = 30000, 1000000, 50, 1e-5
     init = np.random.default_rng(1234).standard_normal(nc)
     opts = irlba.Options(extra_work=20, initial=init, convergence_tolerance=tol)
-    _say("cfg3: mixed top-50 pca")
+    _say("sparse pca 30000x1e6: mixed top-50")
     M = irlba.generate_sparse(nr, nc, 0.03, 42, storage="f32", accumulate="f32")
     mixed = irlba.pca(M, True, True, k, opts)
     del M
     _say(f"  mixed: converged={mixed.converged}, {mixed.iterations} restarts")
-    _say("cfg3: FP64 top-51 pca (the 51st value bounds the gap below the 50th)")
+    _say("sparse pca 30000x1e6: FP64 top-51 (the 51st value bounds the gap below the 50th)")
     A = irlba.generate_sparse(nr, nc, 0.03, 42, storage="f64", accumulate="f64")
     f64 = irlba.pca(A, True, True, k + 1, opts)
     _say(f"  fp64: converged={f64.converged}, {f64.iterations} restarts")
@@ -176,7 +176,7 @@ def test_dense_200000x2000_bf16_fp32_fp64_vs_lapack():
         adj = torch.linalg.norm(Xd.t() @ Ut - Vt * Dt, dim=0).max().item()
         del Xd, Ut, Vt, X
         torch.cuda.empty_cache()
-        _say(f"cfg4 {name}: {res.iterations} restarts, converged={res.converged}, max rel sigma err vs LAPACK on the "
+        _say(f"dense 200000x2000 {name}: {res.iterations} restarts, converged={res.converged}, max rel sigma err vs LAPACK on the "
              f"stored matrix {rel:.3e}, residuals fwd {fwd / ref[0]:.2e} adj {adj / ref[0]:.2e} of s_1")
         assert res.converged
         assert rel < bars[name]
@@ -189,12 +189,12 @@ def test_dense_200000x2000_bf16_fp32_fp64_vs_lapack():
         vals[name] = res
     # Mixed precision against the FP64 run, the benchmark's report.
     for name in ("f32", "bf16"):
-        _say(f"cfg4 {name} vs f64: sigma {np.max(np.abs(vals[name].D - vals['f64'].D) / vals['f64'].D):.3e}, "
+        _say(f"dense 200000x2000 {name} vs f64: sigma {np.max(np.abs(vals[name].D - vals['f64'].D) / vals['f64'].D):.3e}, "
              f"subspace angle {subspace_angle(vals[name].V, vals['f64'].V):.3e} rad")
 
 
 # ---------------------------------------------------------------- backward, dense 4096 x 1024
-def _dense_cfg5():
+def _dense_4096x1024():
     dev = torch.device("cuda:0")
     gen = torch.Generator(device=dev)
     gen.manual_seed(42)
@@ -212,7 +212,7 @@ def _loss(U, S, V, w, GU, GV):
 
 
 def test_backward_dense_4096x1024_vs_oracle_and_fd():
-    A32, w, GU, GV = _dense_cfg5()
+    A32, w, GU, GV = _dense_4096x1024()
     m, n, k = A32.shape[0], A32.shape[1], w.size
     dev = A32.device
     A = A32.double()  # the FP32 values, exactly
@@ -228,7 +228,7 @@ def test_backward_dense_4096x1024_vs_oracle_and_fd():
         Xo, Yo = OB.svd_backward(An, Un, Sn, Vn, w, GU, GV, comp)
         ref = Xo @ Yo.T
         err = np.abs(At.grad.cpu().numpy() - ref).max() / np.abs(ref).max()
-        _say(f"cfg5 dense {comp}: max rel gradient difference vs the oracle {err:.3e}")
+        _say(f"backward dense 4096x1024 {comp}: max rel gradient difference vs the oracle {err:.3e}")
         # FP64 storage: the oracle's full-SVD derivative to 1e-8 of the largest entry.
         assert err < 1e-8
         grads[comp] = (At.grad.cpu().numpy(), Un, Vn)
@@ -241,7 +241,7 @@ def test_backward_dense_4096x1024_vs_oracle_and_fd():
     Lm = OB.loss(An - h * D, k, w, GU, GV, Un, Vn)
     fd = (Lp - Lm) / (2 * h)
     an = float(np.sum(G * D))
-    _say(f"cfg5 dense directional derivative: analytic {an:.10e}, central difference {fd:.10e}")
+    _say(f"backward dense 4096x1024 directional derivative: analytic {an:.10e}, central difference {fd:.10e}")
     # h = 2e-6 on gaps >= ~9 between the planted values (>= ~89 to the noise):
     # the O(h^2) term and the LAPACK rounding over 2h stay below 1e-5 relative.
     assert abs(fd - an) <= 1e-5 * max(1.0, abs(an))
@@ -252,12 +252,12 @@ def test_backward_dense_4096x1024_vs_oracle_and_fd():
     _loss(U, S, V, tw, tGU * torch.tensor(sg, device=dev), tGV * torch.tensor(sg, device=dev)).backward()
     g32 = At.grad.double().cpu().numpy()
     err = np.abs(g32 - G).max() / np.abs(G).max()
-    _say(f"cfg5 dense FP32 storage vs FP64: {err:.3e}")
+    _say(f"backward dense 4096x1024 FP32 storage vs FP64: {err:.3e}")
     assert err < 1e-4
 
 
 # ---------------------------------------------------------------- backward, sparse 50000 x 20000
-def _sparse_cfg5(dev):
+def _sparse_50000x20000(dev):
     gen = torch.Generator(device=dev)
     gen.manual_seed(42)
     m, n, density = 50000, 20000, 0.01
@@ -288,12 +288,12 @@ def _pattern_product(U, w, V, idx, ptr, chunk=2_000_000):
 
 def test_backward_sparse_50000x20000_values_loss():
     dev = torch.device("cuda:0")
-    m, n, vals32, idx, ptr = _sparse_cfg5(dev)
+    m, n, vals32, idx, ptr = _sparse_50000x20000(dev)
     k = 16
     w = np.random.default_rng(8).standard_normal(k)
     tw = torch.tensor(w, device=dev)
     idx_h, ptr_h = idx.cpu().numpy(), ptr.cpu().numpy()
-    _say(f"cfg5 sparse: {m}x{n}, {idx.numel()} stored entries")
+    _say(f"backward sparse 50000x20000: {m}x{n}, {idx.numel()} stored entries")
     opts = irlba.Options(convergence_tolerance=1e-10, extra_work=30)
     # L = sum w_i s_i: dL/dA = U diag(w) V' (no complement terms), on the pattern.
     # FP64 storage to rounding (1e-12 of the largest entry); FP32 storage (FP32
