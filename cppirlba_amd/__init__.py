@@ -278,6 +278,15 @@ class _Handle:
         check(lib().irlba_hip_sparse_layout(self.handle, int(adjoint), ctypes.byref(out)))
         return {name: getattr(out, name) for name, _ in Layout._fields_}
 
+    def value_dictionary(self, enable=True):
+        """Sparse handles: stream one code byte per stored value, looked up in a table of the distinct
+        values, instead of the FP32/FP64 value (lossless; needs at most 256 distinct values and the
+        8-bit delta layout).  Acts on the stored matrix, so on every handle that shares it; returns the
+        table size, 0 after ``enable=False`` (irlba_hip_sparse_value_dictionary)."""
+        n = ctypes.c_int32()
+        check(lib().irlba_hip_sparse_value_dictionary(self.handle, int(bool(enable)), ctypes.byref(n)))
+        return n.value
+
     def realize(self):
         """RealizeWorkspace::realize_copy (interface.hpp:125-130)."""
         nr, nc = self.shape

@@ -60,6 +60,7 @@ class Layout(ctypes.Structure):
         ("lanes_per_line", ctypes.c_int32),
         ("delta", ctypes.c_int32),
         ("blocks", ctypes.c_int32),
+        ("dictionary", ctypes.c_int32),
     ]
 
 
@@ -144,4 +145,5 @@ SIGNATURES.update({
     "irlba_hip_sddmm_noalloc": (_i, [_VP, _i64, _i64, _VP, _VP, _i, _i64, _VP, _VP, _VP, _i]),
     "irlba_hip_multiply_block": (_i, [_VP, _i, _i64, _VP, _VP, _i]),
     "irlba_hip_sparse_layout": (_i, [_VP, _i, ctypes.POINTER(Layout)]),
+    "irlba_hip_sparse_value_dictionary": (_i, [_VP, _i, ctypes.POINTER(ctypes.c_int32)]),
 })

@@ -202,7 +202,18 @@ struct TiledLines {
     bool delta = false;
     std::uint8_t* dlt = nullptr;
     std::uint16_t* first = nullptr;  // device, n_tiles*n_major
+    // Value dictionary (delta layout only; kernels_sparse.hip, "value
+    // dictionary"): {4 codes, 4 deltas} per 4 slots and the table of values
+    // (storage type, n_dict entries).  The products read them when use_dict is
+    // set; the value stream stays for the other kernels.
+    const void* cd = nullptr;
+    const void* dict = nullptr;
+    int n_dict = 0;
+    bool use_dict = false;
+    int lpl_values = 0, lpl_dict = 0;  // lanes per line tuned for each stream
 };
+
+constexpr int kDictMaxCodes = 256;
 
 // ------------------------------------------------------------------ launchers (kernels_*.hip)
 // Sparse build.
@@ -214,6 +225,10 @@ void launch_scatter_entries(const std::int32_t* idx, const void* val_in, Storage
                             i64 n_primary, bool transpose, int tile_len, i64 n_major_out,
                             unsigned long long* cursor, std::uint16_t* out_idx, void* out_val, StorageType out_t,
                             hipStream_t s);
+void launch_dict_collect(const void* val, StorageType st, i64 n, unsigned long long* gset, int* gcount, int* flag,
+                         hipStream_t s);
+void launch_dict_encode(const TiledLines& t, const unsigned long long* pats, int n_codes, u32x2* cd, int* flag,
+                        hipStream_t s);
 void launch_sort_segments(const i64* ptr, const unsigned long long* counts, i64 n_seg, int tile_len,
                           const std::uint16_t* in_idx, const void* in_val, std::uint16_t* out_idx, void* out_val,
                           StorageType st, int* dup_flag, hipStream_t s);

@@ -575,8 +575,9 @@ TiledLines build_lines(irlba_hip_context* ctx, const std::int32_t* d_idx, const 
 }
 
 // Lanes per line: the best group width depends on the segment lengths, the
-// number of tiles and the precision, so each orientation times a plain product
-// at 8, 16, 32 and 64 lanes (HIP events, best of two) and keeps the fastest.
+// number of tiles, the precision and the stream (values or dictionary codes),
+// so each orientation times a plain product at 8, 16, 32 and 64 lanes (4 too
+// for codes; HIP events, best of two) and keeps the fastest.
 void autotune_lanes(irlba_hip_context* ctx, TiledLines& t, bool acc64, void* partial, double* gamma_part) {
     if (std::getenv("IRLBA_HIP_LPL") || t.nnz == 0 || t.n_major == 0) return;
     if (t.nnz < (static_cast<i64>(1) << 20)) return;  // small matrices: launch-bound, keep the heuristic
@@ -593,7 +594,11 @@ void autotune_lanes(irlba_hip_context* ctx, TiledLines& t, bool acc64, void* par
     IRLBA_HIP_CHECK(hipEventCreate(&e1));
     float best = 0;
     int best_lpl = t.lpl;
-    for (int lpl : {8, 16, 32, 64}) {
+    // The dictionary stream (2 bytes per slot) also tries 4 lanes per line,
+    // which suits its many-tile FP64 orientation.
+    static const int kPlain[] = {8, 16, 32, 64}, kDict[] = {4, 8, 16, 32, 64};
+    const bool dict = t.use_dict && t.cd;
+    for (int lpl : dict ? std::vector<int>(kDict, kDict + 5) : std::vector<int>(kPlain, kPlain + 4)) {
         t.lpl = lpl;
         launch_spmv_tiled(t, pro, epi, acc64, partial, gamma_part, s);  // warm-up
         float ms_best = 0;
@@ -688,6 +693,112 @@ std::shared_ptr<SparseBase> make_sparse(irlba_hip_context* ctx, i64 nrow, i64 nc
     autotune_lanes(ctx, sp->by_row, acc64, sp->partial_row.get(), sp->gamma_row.get());
     autotune_lanes(ctx, sp->by_col, acc64, sp->partial_col.get(), sp->gamma_col.get());
     return sp;
+}
+
+// ------------------------------------------------------------------ value dictionary
+// Switches the products of both orientations between the value stream and one
+// code byte per slot (kernels_sparse.hip, "value dictionary").  The codes are
+// built on the first switch-on: the distinct bit patterns of the stored values
+// (at most 256, +0.0 included for the padding and bridging slots), numbered in
+// increasing order of value so that the table does not depend on the order in
+// which blocks met them.  Returns the table size (0 when switched off).
+int set_value_dictionary(irlba_hip_context* ctx, SparseBase& sb, bool on) {
+    TiledLines* ts[2] = {&sb.by_row, &sb.by_col};
+    if (!on) {
+        for (TiledLines* t : ts) {
+            if (t->use_dict && t->lpl_values) t->lpl = t->lpl_values;
+            t->use_dict = false;
+        }
+        return 0;
+    }
+    for (TiledLines* t : ts)
+        require(t->delta && t->slots > 0,
+                "the value dictionary needs the 8-bit delta layout in both orientations (lines of about 12 or more "
+                "entries per tile)");
+    if (sb.by_row.cd && sb.by_col.cd) {
+        for (TiledLines* t : ts) {
+            if (!t->use_dict && t->lpl_dict) t->lpl = t->lpl_dict;
+            t->use_dict = true;
+        }
+        return sb.by_row.n_dict;
+    }
+    hipStream_t s = ctx->stream;
+    const StorageType st = sb.by_row.st;
+    const bool f64 = st == ST_F64;
+    constexpr int kSet = 1 << 10;  // the global hash set of k_dict_collect
+    auto gset = dalloc<unsigned long long>(kSet);
+    auto cf = dalloc<int>(2);  // distinct count, flags
+    IRLBA_HIP_CHECK(hipMemsetAsync(gset.get(), 0xff, sizeof(unsigned long long) * kSet, s));
+    IRLBA_HIP_CHECK(hipMemsetAsync(cf.get(), 0, 2 * sizeof(int), s));
+    for (TiledLines* t : ts) launch_dict_collect(t->val, t->st, t->slots, gset.get(), cf.get(), cf.get() + 1, s);
+    std::vector<unsigned long long> hset(kSet);
+    int hcf[2] = {0, 0};
+    IRLBA_HIP_CHECK(hipMemcpyAsync(hset.data(), gset.get(), sizeof(unsigned long long) * kSet, hipMemcpyDeviceToHost, s));
+    IRLBA_HIP_CHECK(hipMemcpyAsync(hcf, cf.get(), sizeof(hcf), hipMemcpyDeviceToHost, s));
+    IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+    const char* too_many = "the matrix values take more than 256 distinct values: no value dictionary";
+    require(!(hcf[1] & 1), too_many);
+    std::vector<unsigned long long> pats;
+    for (unsigned long long p : hset)
+        if (p != ~0ull) pats.push_back(p);
+    if (std::find(pats.begin(), pats.end(), 0ull) == pats.end()) pats.push_back(0ull);
+    require(pats.size() <= static_cast<size_t>(kDictMaxCodes), too_many);
+    auto as_value = [f64](unsigned long long p) {
+        if (f64) {
+            double d;
+            std::memcpy(&d, &p, sizeof(d));
+            return d;
+        }
+        const std::uint32_t u = static_cast<std::uint32_t>(p);
+        float f;
+        std::memcpy(&f, &u, sizeof(f));
+        return static_cast<double>(f);
+    };
+    std::sort(pats.begin(), pats.end(), [&](unsigned long long a, unsigned long long b) {
+        const double x = as_value(a), y = as_value(b);
+        const bool nx = std::isnan(x), ny = std::isnan(y);
+        if (nx != ny) return ny;
+        if (!nx && x != y) return x < y;
+        return a < b;
+    });
+    const int n = static_cast<int>(pats.size());
+    const size_t vb = storage_bytes(st);
+    std::vector<unsigned char> table(vb * n);
+    for (int c = 0; c < n; ++c) {
+        if (f64) {
+            std::memcpy(table.data() + 8 * c, &pats[c], 8);
+        } else {
+            const std::uint32_t u = static_cast<std::uint32_t>(pats[c]);
+            std::memcpy(table.data() + 4 * c, &u, 4);
+        }
+    }
+    auto dpats = dalloc<unsigned long long>(n);
+    auto dtab = dalloc_bytes(vb * n);
+    stream_copy(ctx, dpats.get(), pats.data(), sizeof(unsigned long long) * n, hipMemcpyHostToDevice);
+    stream_copy(ctx, dtab.get(), table.data(), vb * n, hipMemcpyHostToDevice);
+    std::shared_ptr<void> cds[2];
+    for (int o = 0; o < 2; ++o) {
+        cds[o] = dalloc_bytes(sizeof(u32x2) * static_cast<size_t>(ts[o]->slots / 4));
+        launch_dict_encode(*ts[o], dpats.get(), n, static_cast<u32x2*>(cds[o].get()), cf.get() + 1, s);
+    }
+    IRLBA_HIP_CHECK(hipMemcpyAsync(hcf, cf.get(), sizeof(hcf), hipMemcpyDeviceToHost, s));
+    IRLBA_HIP_CHECK(hipStreamSynchronize(s));
+    require(!(hcf[1] & 2), "internal error: a stored value is missing from the value dictionary");
+    void* partial[2] = {sb.partial_row.get(), sb.partial_col.get()};
+    double* gamma[2] = {sb.gamma_row.get(), sb.gamma_col.get()};
+    for (int o = 0; o < 2; ++o) {
+        TiledLines* t = ts[o];
+        t->cd = cds[o].get();
+        t->dict = dtab.get();
+        t->n_dict = n;
+        t->lpl_values = t->lpl;
+        t->use_dict = true;
+        autotune_lanes(ctx, *t, sb.acc64, partial[o], gamma[o]);
+        t->lpl_dict = t->lpl;
+        sb.keep.push_back(cds[o]);
+    }
+    sb.keep.push_back(dtab);
+    return n;
 }
 
 // The host FP64 SVD (HostMat, host_svd) lives in host_svd.cpp.
@@ -1756,6 +1867,18 @@ int irlba_hip_sparse_layout(const irlba_hip_matrix* m, int adjoint, irlba_hip_la
         out->lanes_per_line = t.lpl;
         out->delta = t.delta ? 1 : 0;
         out->blocks = t.B;
+        out->dictionary = t.use_dict ? t.n_dict : 0;
+    });
+}
+
+int irlba_hip_sparse_value_dictionary(irlba_hip_matrix* m, int enable, int32_t* n_codes) {
+    return guarded([&] {
+        require(m != nullptr, "null matrix");
+        auto* sb = dynamic_cast<SparseBase*>(m->base.get());
+        require(sb != nullptr, "not a sparse matrix handle");
+        CtxGuard g(m->ctx);
+        const int n = set_value_dictionary(m->ctx, *sb, enable != 0);
+        if (n_codes) *n_codes = n;
     });
 }
 

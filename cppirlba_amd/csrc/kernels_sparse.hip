@@ -399,6 +399,148 @@ void launch_delta_write(const i64* ptrA, const unsigned long long* counts, i64 n
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
+// ------------------------------------------------------------------ value dictionary
+// Values that take at most 256 distinct bit patterns (count data: log1p of
+// small integers, the integers themselves) can stream as one code byte per
+// slot: the products then read {4 codes, 4 deltas} as one 8-byte word per lane
+// step and look the values up in an LDS copy of the table.  The table holds
+// the stored values themselves, so every product term is the same number as
+// with the value stream.  Distinct values are found by bit pattern (so -0.0
+// and +0.0 are two entries) in per-block LDS hash sets merged into a global one.
+
+constexpr unsigned long long kDictEmpty = ~0ull;  // no FP32 pattern widens to it; an FP64 NaN payload could
+constexpr int kDictLocalBits = 9, kDictGlobalBits = 10;
+
+__device__ __forceinline__ unsigned dict_hash(unsigned long long p, int bits) {
+    return static_cast<unsigned>((p * 0x9E3779B97F4A7C15ull) >> (64 - bits));
+}
+__device__ __forceinline__ unsigned long long value_bits(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ unsigned long long value_bits(double v) {
+    return static_cast<unsigned long long>(__double_as_longlong(v));
+}
+
+// Inserts p into a linear-probing set of 2^bits slots; returns 1 if p was new,
+// 0 if present, -1 if the set is full.  Present entries are found by plain
+// (relaxed atomic) reads, so the common case takes no atomic operation.
+template <int BITS>
+__device__ __forceinline__ int dict_insert(unsigned long long* set, unsigned long long p) {
+    constexpr unsigned mask = (1u << BITS) - 1;
+    unsigned h = dict_hash(p, BITS);
+    for (int probe = 0; probe <= static_cast<int>(mask); ++probe, h = (h + 1) & mask) {
+        unsigned long long cur = __hip_atomic_load(set + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == p) return 0;
+        if (cur == kDictEmpty) {
+            cur = atomicCAS(set + h, kDictEmpty, p);
+            if (cur == kDictEmpty) return 1;
+            if (cur == p) return 0;
+        }
+    }
+    return -1;
+}
+
+// flag bit 1: more than 256 distinct patterns (or the reserved pattern) seen.
+template <typename VT>
+__global__ __launch_bounds__(256) void k_dict_collect(const VT* __restrict__ val, i64 n,
+                                                      unsigned long long* __restrict__ gset, int* __restrict__ gcount,
+                                                      int* __restrict__ flag) {
+    __shared__ unsigned long long lset[1 << kDictLocalBits];
+    __shared__ int lcount;
+    for (int j = threadIdx.x; j < (1 << kDictLocalBits); j += blockDim.x) lset[j] = kDictEmpty;
+    if (threadIdx.x == 0) lcount = 0;
+    __syncthreads();
+    unsigned long long last = kDictEmpty;  // this thread's previous pattern: runs of one value skip the set
+    const i64 stride = static_cast<i64>(gridDim.x) * blockDim.x;
+    for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const unsigned long long p = value_bits(__builtin_nontemporal_load(val + i));
+        if (p == last) continue;
+        last = p;
+        if (p == kDictEmpty) {
+            atomicOr(flag, 1);
+            break;
+        }
+        const int r = dict_insert<kDictLocalBits>(lset, p);
+        if (r < 0 || (r == 1 && atomicAdd(&lcount, 1) >= 256)) atomicOr(flag, 1);
+        if (__hip_atomic_load(&lcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > 256) break;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < (1 << kDictLocalBits); j += blockDim.x) {
+        const unsigned long long p = lset[j];
+        if (p == kDictEmpty) continue;
+        const int r = dict_insert<kDictGlobalBits>(gset, p);
+        if (r < 0 || (r == 1 && atomicAdd(gcount, 1) >= 256)) atomicOr(flag, 1);
+    }
+}
+
+void launch_dict_collect(const void* val, StorageType st, i64 n, unsigned long long* gset, int* gcount, int* flag,
+                         hipStream_t s) {
+    if (n <= 0) return;
+    const dim3 g(static_cast<unsigned>(std::max<i64>(1, std::min<i64>((n + 255) / 256, 4096)))), b(256);
+    if (st == ST_F64)
+        hipLaunchKernelGGL(k_dict_collect<double>, g, b, 0, s, static_cast<const double*>(val), n, gset, gcount, flag);
+    else if (st == ST_F32)
+        hipLaunchKernelGGL(k_dict_collect<float>, g, b, 0, s, static_cast<const float*>(val), n, gset, gcount, flag);
+    else
+        throw std::runtime_error("the value dictionary holds FP64 or FP32 values");
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
+// cd[q] = {codes of slots 4q .. 4q+3 (byte k = slot 4q+k), their four delta
+// bytes}.  pats[c] is the bit pattern of code c (n_codes distinct patterns).
+// flag bit 2: a slot value missing from the table.
+template <typename VT>
+__global__ __launch_bounds__(256) void k_dict_encode(const VT* __restrict__ val, const unsigned* __restrict__ dlt4,
+                                                     i64 nquads, const unsigned long long* __restrict__ pats,
+                                                     int n_codes, u32x2* __restrict__ cd, int* __restrict__ flag) {
+    __shared__ unsigned long long kset[1 << kDictLocalBits];
+    __shared__ unsigned char kcode[1 << kDictLocalBits];
+    constexpr unsigned mask = (1u << kDictLocalBits) - 1;
+    for (int j = threadIdx.x; j < (1 << kDictLocalBits); j += blockDim.x) kset[j] = kDictEmpty;
+    __syncthreads();
+    if (threadIdx.x == 0) {  // at most 256 patterns, distinct: a serial fill is simplest
+        for (int c = 0; c < n_codes; ++c) {
+            unsigned h = dict_hash(pats[c], kDictLocalBits);
+            while (kset[h] != kDictEmpty) h = (h + 1) & mask;
+            kset[h] = pats[c];
+            kcode[h] = static_cast<unsigned char>(c);
+        }
+    }
+    __syncthreads();
+    const i64 stride = static_cast<i64>(gridDim.x) * blockDim.x;
+    for (i64 q = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; q < nquads; q += stride) {
+        unsigned codes = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const unsigned long long p = value_bits(val[4 * q + k]);
+            unsigned h = dict_hash(p, kDictLocalBits);
+            int probe = 0;
+            while (kset[h] != p && kset[h] != kDictEmpty && probe <= static_cast<int>(mask)) {
+                h = (h + 1) & mask;
+                ++probe;
+            }
+            if (kset[h] == p) codes |= static_cast<unsigned>(kcode[h]) << (8 * k);
+            else atomicOr(flag, 2);
+        }
+        cd[q] = u32x2{codes, dlt4[q]};
+    }
+}
+
+void launch_dict_encode(const TiledLines& t, const unsigned long long* pats, int n_codes, u32x2* cd, int* flag,
+                        hipStream_t s) {
+    const i64 nquads = t.slots / 4;
+    if (nquads <= 0) return;
+    const dim3 g(static_cast<unsigned>(std::max<i64>(1, std::min<i64>((nquads + 255) / 256, 8192)))), b(256);
+    const unsigned* d4 = reinterpret_cast<const unsigned*>(t.dlt);
+    if (t.st == ST_F64)
+        hipLaunchKernelGGL(k_dict_encode<double>, g, b, 0, s, static_cast<const double*>(t.val), d4, nquads, pats,
+                           n_codes, cd, flag);
+    else if (t.st == ST_F32)
+        hipLaunchKernelGGL(k_dict_encode<float>, g, b, 0, s, static_cast<const float*>(t.val), d4, nquads, pats,
+                           n_codes, cd, flag);
+    else
+        throw std::runtime_error("the value dictionary holds FP64 or FP32 values");
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
 void launch_delta_expand(const TiledLines& t, std::uint16_t* idx, hipStream_t s) {
     const i64 nseg = static_cast<i64>(t.n_tiles) * t.n_major;
     hipLaunchKernelGGL(k_delta_expand, dim3(static_cast<unsigned>(per_segment_grid(nseg))), dim3(256), 0, s, t.ptr,
@@ -415,6 +557,9 @@ struct SpmvParams {
     const i64* chunk;
     const std::uint8_t* dlt;      // 8-bit delta indices (delta layout) ...
     const std::uint16_t* first;   // ... and the first index of each segment
+    const u32x2* cd;              // value dictionary: {4 codes, 4 deltas} per lane step ...
+    const void* dict;             // ... and the table (VT), n_dict entries
+    int n_dict;
     i64 n_major, n_minor;
     int tile_len, n_tiles, B;
     Prologue pro;
@@ -492,7 +637,7 @@ __device__ __forceinline__ int xcd_chunk(int b, int nb) {
     return (nb % kXcds == 0) ? (b % kXcds) * (nb / kXcds) + b / kXcds : b;
 }
 
-template <typename VT, typename AT, int LPL, int VEC, bool DELTA>
+template <typename VT, typename AT, int LPL, int VEC, bool DELTA, bool DICT>
 __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
     extern __shared__ unsigned char smem_raw[];
     AT* xs = reinterpret_cast<AT*>(smem_raw);
@@ -509,10 +654,19 @@ __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
     constexpr int STRIDE = LPL * VEC;
     // Steps in flight per lane: the tile in LDS allows one 16-wave block per
     // CU, so up to 128 VGPRs cost no occupancy; FP64 values or sums fit four steps.
-    constexpr int UNR = (sizeof(VT) == 8 || sizeof(AT) == 8) ? 4 : 8;
+    // Dictionary codes hold four registers per step whatever the precision.
+    constexpr int UNR = DICT ? 8 : ((sizeof(VT) == 8 || sizeof(AT) == 8) ? 4 : 8);
     const Prologue& pro = p.pro;
     const bool single = p.n_tiles == 1;
     const double coef = (single && p.epi.axpy_src) ? *p.epi.axpy_coef : 0.0;
+    // The value table sits after the input tile; the first tile's barrier
+    // publishes it.
+    [[maybe_unused]] AT* tab = xs + p.tile_len;
+    if constexpr (DICT) {
+        static_assert(DELTA, "the value dictionary is a form of the delta layout");
+        const VT* dict = static_cast<const VT*>(p.dict);
+        for (int j = threadIdx.x; j < p.n_dict; j += blockDim.x) tab[j] = static_cast<AT>(dict[j]);
+    }
 
     for (i64 t = t_first; t <= t_last; ++t) {
     if (t > t_first) __syncthreads();  // every group is done with the previous tile
@@ -586,7 +740,77 @@ __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
             // 1.50 -> 1.40 ms (with the lanes per line tuned for each form); a
             // form with each lane decoding its own two pairs (one scan of
             // packed pair sums, 16-bit delta loads) measured 1.77 / 1.46 ms.
-            constexpr bool PAIRS = sizeof(VT) == 8;
+            constexpr bool PAIRS = sizeof(VT) == 8 && !DICT;
+            if constexpr (DICT) {
+                // One 8-byte word per lane step: the codes and deltas of its
+                // four slots (lane l: slots 4l .. 4l+3 of the step).  Whole
+                // iterations inside the segment run without predicates (no
+                // compares, masks or branches: the product is bound by
+                // instruction issue, not by its 2 bytes per slot), then one
+                // predicated tail.
+                const u32x2* __restrict__ cp = p.cd + (s0 >> 2) + lane_in;
+                int left = static_cast<int>(s1 - s0);
+                for (; left >= UNR * STRIDE; left -= UNR * STRIDE, cp += UNR * LPL) {
+                    unsigned cw[UNR], dw[UNR], tot[UNR], inc[UNR];
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        const u32x2 w = __builtin_nontemporal_load(cp + u * LPL);
+                        cw[u] = w.x;
+                        dw[u] = w.y;
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        tot[u] = __builtin_amdgcn_sad_u8(dw[u], 0u, 0u);
+                        inc[u] = group_scan_incl_dpp<LPL>(tot[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        unsigned q[4];
+                        delta_indices(dw[u], carry, inc[u], tot[u], q);
+                        carry += group_last<LPL>(inc[u]);
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const AT v = tab[(cw[u] >> (8 * k)) & 0xffu];
+                            if (u & 1) acc1 += v * xs[q[k]];
+                            else acc0 += v * xs[q[k]];
+                        }
+                    }
+                }
+                if (left > 0) {
+                    unsigned cw[UNR], dw[UNR], tot[UNR], inc[UNR];
+                    bool ok[UNR];
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        ok[u] = lane_in * 4 + u * STRIDE < left;
+                        cw[u] = 0;
+                        dw[u] = 0;
+                        if (ok[u]) {
+                            const u32x2 w = __builtin_nontemporal_load(cp + u * LPL);
+                            cw[u] = w.x;
+                            dw[u] = w.y;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        tot[u] = __builtin_amdgcn_sad_u8(dw[u], 0u, 0u);
+                        inc[u] = group_scan_incl_dpp<LPL>(tot[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        unsigned q[4];
+                        delta_indices(dw[u], carry, inc[u], tot[u], q);
+                        carry += group_last<LPL>(inc[u]);
+                        if (ok[u]) {
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) {
+                                const AT v = tab[(cw[u] >> (8 * k)) & 0xffu];
+                                if (u & 1) acc1 += v * xs[q[k]];
+                                else acc0 += v * xs[q[k]];
+                            }
+                        }
+                    }
+                }
+            } else
             for (i64 sb = s0; sb < s1; sb += UNR * STRIDE) {
                 VT v[UNR][4];
                 unsigned dw[UNR], tot[UNR], inc[UNR];
@@ -747,6 +971,9 @@ static void spmv_launch_one(const TiledLines& tl, const Prologue& pro, const Epi
     p.chunk = tl.chunk;
     p.dlt = tl.dlt;
     p.first = tl.first;
+    p.cd = static_cast<const u32x2*>(tl.cd);
+    p.dict = tl.dict;
+    p.n_dict = tl.n_dict;
     p.n_major = tl.n_major;
     p.n_minor = tl.n_minor;
     p.tile_len = tl.tile_len;
@@ -756,13 +983,18 @@ static void spmv_launch_one(const TiledLines& tl, const Prologue& pro, const Epi
     p.epi = epi;
     p.partial = static_cast<AT*>(partial);
     p.gamma_part = gamma_part;
-    const size_t lds = static_cast<size_t>(tl.tile_len) * sizeof(AT);
+    const bool dict = tl.use_dict && tl.cd && tl.delta;
+    const size_t lds = (static_cast<size_t>(tl.tile_len) + (dict ? kDictMaxCodes : 0)) * sizeof(AT);
     if (ev0) IRLBA_HIP_CHECK(hipEventRecord(ev0, s));
     if constexpr (VEC == 4) {
-        if (tl.delta) hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, 4, true>), dim3(tl.B), dim3(1024), lds, s, p);
-        else hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, 4, false>), dim3(tl.B), dim3(1024), lds, s, p);
+        if (dict)
+            hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, 4, true, true>), dim3(tl.B), dim3(1024), lds, s, p);
+        else if (tl.delta)
+            hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, 4, true, false>), dim3(tl.B), dim3(1024), lds, s, p);
+        else
+            hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, 4, false, false>), dim3(tl.B), dim3(1024), lds, s, p);
     } else {
-        hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, VEC, false>), dim3(tl.B), dim3(1024), lds, s, p);
+        hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, VEC, false, false>), dim3(tl.B), dim3(1024), lds, s, p);
     }
     IRLBA_HIP_LAUNCH_CHECK();
     if (ev1) IRLBA_HIP_CHECK(hipEventRecord(ev1, s));

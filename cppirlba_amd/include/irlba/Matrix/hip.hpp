@@ -322,6 +322,19 @@ public:
         return my_part->nonzero;
     }
 
+    // No reference counterpart (a storage choice behind the value array): the
+    // products stream one code byte per stored value, looked up in a table of
+    // the distinct values, instead of the value itself.  Lossless; needs at
+    // most 256 distinct values (count data) and throws otherwise.  Applies to
+    // every decorator built on this matrix.  Returns the table size (0 when
+    // switched off).
+    int use_value_dictionary(bool enable = true) {
+        std::int32_t n = 0;
+        hip_check(irlba_hip_sparse_value_dictionary(const_cast<irlba_hip_matrix*>(this->device_handle()),
+                                                     enable ? 1 : 0, &n));
+        return n;
+    }
+
 private:
     ValueArray_ my_values;
     IndexArray_ my_indices;

@@ -139,8 +139,24 @@ typedef struct irlba_hip_layout {
     int32_t lanes_per_line;
     int32_t delta;          /* 1: 8-bit delta indices */
     int32_t blocks;         /* workgroups of one product */
+    int32_t dictionary;     /* entries of the value dictionary the products read, 0: the value stream */
 } irlba_hip_layout;
 int irlba_hip_sparse_layout(const irlba_hip_matrix* m, int adjoint, irlba_hip_layout* out);
+
+/* Value dictionary for a sparse handle (no reference counterpart: a storage
+ * option of ParallelSparseMatrix's value array, include/irlba/Matrix/sparse.hpp:469,
+ * whose products read values through ValueArray_'s operator[]).  enable = 1
+ * makes the products of both orientations stream one code byte per stored slot
+ * instead of the FP32/FP64 value, looked up in a table of the distinct stored
+ * values; lossless, every product term is the same number.  Needs the 8-bit
+ * delta layout in both orientations and at most 256 distinct value bit
+ * patterns (count data); otherwise returns an error and the products keep the
+ * value stream.  The codes are built on the first call (2 more bytes of HBM per
+ * slot per orientation; the value stream stays for realize, column statistics
+ * and block products) and re-used after; enable = 0 switches back.  It acts on
+ * the stored matrix, so on every handle that shares it (decorators).
+ * *n_codes receives the table size (0 when switched off). */
+int irlba_hip_sparse_value_dictionary(irlba_hip_matrix* m, int enable, int32_t* n_codes);
 
 /* Workspace::multiply (adjoint = 0) and AdjointWorkspace::multiply (adjoint = 1)
  * (include/irlba/Matrix/interface.hpp:49,87): output is fully overwritten. */

@@ -79,12 +79,14 @@ def test_null_handles_report_errors():
     L = lib()
     i64 = ctypes.c_int64
     r, c = i64(), i64()
+    n32 = ctypes.c_int32()
     st = Stats()
     calls = {
         "context_synchronize": lambda: L.irlba_hip_context_synchronize(None),
         "get_stats": lambda: L.irlba_hip_get_stats(None, ctypes.byref(st)),
         "set_profiling": lambda: L.irlba_hip_context_set_profiling(None, 1),
         "matrix_dims": lambda: L.irlba_hip_matrix_dims(None, ctypes.byref(r), ctypes.byref(c)),
+        "sparse_value_dictionary": lambda: L.irlba_hip_sparse_value_dictionary(None, 1, ctypes.byref(n32)),
     }
     for name, call in calls.items():
         assert call() != 0, name
