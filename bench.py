@@ -317,6 +317,28 @@ def dense_leg(args):
     return res
 
 
+def _dict_leg(mat, res, dt, st, nr, nc):
+    """Solve time, per-product times and the roofline of the A.v product for a
+    matrix whose products stream the value-dictionary codes."""
+    import numpy as np
+
+    fwd = st["forward_kernel_ms"] / max(1, st["forward_timed"])
+    adj = st["adjoint_kernel_ms"] / max(1, st["adjoint_timed"])
+    lay = mat.layout(False)
+    nseg = lay["n_tiles"] * nr
+    # {code, delta} bytes per slot, segment pointers and first indices, FP64 vectors
+    stream = int(lay["slots"] * 2 + (nseg + 1) * 8 + nseg * 2 + (nr + nc) * 8)
+    return {
+        "solve_s": round(dt, 4), "iterations": int(res.iterations), "converged": bool(res.converged),
+        "ms_per_restart": round(1e3 * dt / max(1, res.iterations), 3),
+        "forward_ms": round(fwd, 4), "adjoint_ms": round(adj, 4),
+        "forward_lanes_per_line": lay["lanes_per_line"], "adjoint_lanes_per_line": mat.layout(True)["lanes_per_line"],
+        "forward_stream_bytes": stream, "forward_achieved_GBps": round(stream / (fwd * 1e6), 1),
+        "forward_frac": round(stream / (fwd * 1e6) / HBM_PEAK_GBS, 4),
+        "_variances": np.asarray(res.variances),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--mode", choices=["solve", "spmv", "backward", "dense"], default="solve")
@@ -339,6 +361,8 @@ def main():
     ap.add_argument("--dense-cols", type=int, default=2000)
     ap.add_argument("--dense-reps", type=int, default=1)
     ap.add_argument("--dense-storages", default="bf16,f32,f64")
+    ap.add_argument("--skip-dictionary", action="store_true",
+                    help="skip the value-dictionary solves (codes in place of the stored values)")
     ap.add_argument("--skip-backward-exact-sparse", action="store_true",
                     help="time only the rank-k complement on the sparse backward case")
     ap.add_argument("--cpu-sample-cols", type=int, default=33334)
@@ -497,6 +521,25 @@ def main():
                                                  "kernel); not measured in this run")
             out["roofline"]["traffic_frac"] = round(tr[key] / (dom_ms * 1e6) / HBM_PEAK_GBS, 4)
 
+    dict_out = None
+    if not args.skip_dictionary:
+        # The same matrix with its values streamed as one code byte per slot
+        # (a table of the distinct stored values; lossless, and with FP32
+        # storage every product term is the same number).
+        log("value dictionary: mixed solve with the values streamed as codes")
+        t0 = time.perf_counter()
+        n_codes = mixed.value_dictionary(True)
+        ctx.synchronize()
+        build_dict_s = time.perf_counter() - t0
+        res_d, dt_d = solve(irlba, mixed, args.number, opts)
+        st_d = ctx.stats()
+        res_d = host_copy(res_d)
+        log(f"  dictionary of {n_codes} values built in {build_dict_s:.3f} s; mixed solve {dt_d:.3f} s, "
+            f"{res_d.iterations} restarts")
+        dict_out = {"distinct_values": n_codes, "build_s": round(build_dict_s, 3),
+                    "mixed": _dict_leg(mixed, res_d, dt_d, st_d, nr, nc)}
+        mixed.value_dictionary(False)
+
     if not args.skip_fp64:
         del mixed
         log("FP64 path on the same matrix (FP64 storage and accumulation)")
@@ -527,7 +570,32 @@ def main():
             (dt64 / max(1, res64.iterations)) / (float(np.mean(times)) / max(1, mixed_res.iterations)), 3)
         out["sv_rel_err_vs_fp64"] = rel
         out["max_subspace_angle_vs_fp64"] = angle
+        if dict_out is not None:
+            log("value dictionary: FP64 solve with the values streamed as codes")
+            fp64.value_dictionary(True)
+            res_d64, dt_d64 = solve(irlba, fp64, args.number, opts)
+            st_d64 = ctx.stats()
+            res_d64 = host_copy(res_d64)
+            log(f"  fp64 with codes: {dt_d64:.3f} s, {res_d64.iterations} restarts")
+            dict_out["fp64"] = _dict_leg(fp64, res_d64, dt_d64, st_d64, nr, nc)
+            dm = dict_out["mixed"]
+            dict_out["mixed_vs_fp64_per_restart_speedup"] = round(
+                dict_out["fp64"]["ms_per_restart"] / dm["ms_per_restart"], 3)
+            var_m = dm.pop("_variances")
+            var_d = dict_out["fp64"].pop("_variances")
+            # singular values of the coded solves against the value-stream runs
+            dict_out["mixed_sv_rel_err_vs_fp64"] = float(np.max(np.abs(np.sqrt(var_m) - sv_d) / sv_d))
+            dict_out["mixed_sv_rel_err_vs_value_stream_mixed"] = float(np.max(np.abs(np.sqrt(var_m) - sv_m) / sv_m))
+            dict_out["fp64_sv_rel_err_vs_value_stream_fp64"] = float(np.max(np.abs(np.sqrt(var_d) - sv_d) / sv_d))
         del fp64
+    if dict_out is not None:
+        for leg in ("mixed", "fp64"):
+            if leg in dict_out:
+                dict_out[leg].pop("_variances", None)
+        dict_out["note"] = ("irlba_hip_sparse_value_dictionary on the same matrices: the products stream a "
+                            "{1-byte code, 1-byte delta} pair per slot and read the value from an LDS table of the "
+                            "distinct stored values (lossless); roofline bytes are those of that stream")
+        out["value_dictionary"] = dict_out
 
     if not args.skip_backward:
         log("SVD backward (top-16, dense and sparse)")

@@ -61,6 +61,7 @@ class Layout(ctypes.Structure):
         ("delta", ctypes.c_int32),
         ("blocks", ctypes.c_int32),
         ("dictionary", ctypes.c_int32),
+        ("dictionary_run", ctypes.c_int32),
     ]
 
 

@@ -203,13 +203,15 @@ struct TiledLines {
     std::uint8_t* dlt = nullptr;
     std::uint16_t* first = nullptr;  // device, n_tiles*n_major
     // Value dictionary (delta layout only; kernels_sparse.hip, "value
-    // dictionary"): {4 codes, 4 deltas} per 4 slots and the table of values
+    // dictionary"): {codes, deltas} per run of 4 or 16 slots and the table of values
     // (storage type, n_dict entries).  The products read them when use_dict is
     // set; the value stream stays for the other kernels.
     const void* cd = nullptr;
     const void* dict = nullptr;
     int n_dict = 0;
+    int dict_run = 4;                  // slots per code word: 4, or 16 when every segment holds whole runs of 16
     bool use_dict = false;
+    int seg_align = 1;                 // every segment's slot count is a multiple of this
     int lpl_values = 0, lpl_dict = 0;  // lanes per line tuned for each stream
 };
 
@@ -227,7 +229,7 @@ void launch_scatter_entries(const std::int32_t* idx, const void* val_in, Storage
                             hipStream_t s);
 void launch_dict_collect(const void* val, StorageType st, i64 n, unsigned long long* gset, int* gcount, int* flag,
                          hipStream_t s);
-void launch_dict_encode(const TiledLines& t, const unsigned long long* pats, int n_codes, u32x2* cd, int* flag,
+void launch_dict_encode(const TiledLines& t, const unsigned long long* pats, int n_codes, void* cd, int* flag,
                         hipStream_t s);
 void launch_sort_segments(const i64* ptr, const unsigned long long* counts, i64 n_seg, int tile_len,
                           const std::uint16_t* in_idx, const void* in_val, std::uint16_t* out_idx, void* out_val,

@@ -475,6 +475,7 @@ TiledLines build_lines(irlba_hip_context* ctx, const std::int32_t* d_idx, const 
     while (seg_align * 2 <= 64 && seg_align * 2 <= 0.12 * avg) seg_align *= 2;
     if (const char* e = std::getenv("IRLBA_HIP_SEG_ALIGN")) seg_align = std::max(t.vec, std::atoi(e) / t.vec * t.vec);
     launch_pad_counts(counts.get(), plen.get(), nseg, seg_align, s);
+    t.seg_align = seg_align;
     auto ptr = dalloc<i64>(nseg + 1);
     exclusive_scan_i64(plen.get(), ptr.get(), nseg, s);
     plen.reset();
@@ -778,8 +779,11 @@ int set_value_dictionary(irlba_hip_context* ctx, SparseBase& sb, bool on) {
     stream_copy(ctx, dtab.get(), table.data(), vb * n, hipMemcpyHostToDevice);
     std::shared_ptr<void> cds[2];
     for (int o = 0; o < 2; ++o) {
-        cds[o] = dalloc_bytes(sizeof(u32x2) * static_cast<size_t>(ts[o]->slots / 4));
-        launch_dict_encode(*ts[o], dpats.get(), n, static_cast<u32x2*>(cds[o].get()), cf.get() + 1, s);
+        // Runs of 16 slots (one cross-lane index scan per 16 slots instead of
+        // per 4) when every segment holds whole runs.
+        ts[o]->dict_run = ts[o]->seg_align % 16 == 0 ? 16 : 4;
+        cds[o] = dalloc_bytes(2 * static_cast<size_t>(ts[o]->slots));
+        launch_dict_encode(*ts[o], dpats.get(), n, cds[o].get(), cf.get() + 1, s);
     }
     IRLBA_HIP_CHECK(hipMemcpyAsync(hcf, cf.get(), sizeof(hcf), hipMemcpyDeviceToHost, s));
     IRLBA_HIP_CHECK(hipStreamSynchronize(s));
@@ -1868,6 +1872,7 @@ int irlba_hip_sparse_layout(const irlba_hip_matrix* m, int adjoint, irlba_hip_la
         out->delta = t.delta ? 1 : 0;
         out->blocks = t.B;
         out->dictionary = t.use_dict ? t.n_dict : 0;
+        out->dictionary_run = t.use_dict ? t.dict_run : 0;
     });
 }
 

@@ -484,13 +484,15 @@ void launch_dict_collect(const void* val, StorageType st, i64 n, unsigned long l
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
-// cd[q] = {codes of slots 4q .. 4q+3 (byte k = slot 4q+k), their four delta
-// bytes}.  pats[c] is the bit pattern of code c (n_codes distinct patterns).
-// flag bit 2: a slot value missing from the table.
-template <typename VT>
+// Code stream, in runs of RUN = 4 or 16 slots: run r holds the RUN code bytes
+// of slots RUN r .. RUN r + RUN - 1 (byte k = slot RUN r + k), then their RUN
+// delta bytes, 2 RUN bytes in all.  pats[c] is the bit pattern of code c
+// (n_codes distinct patterns).  flag bit 2: a slot value missing from the table.
+template <typename VT, int RUN>
 __global__ __launch_bounds__(256) void k_dict_encode(const VT* __restrict__ val, const unsigned* __restrict__ dlt4,
-                                                     i64 nquads, const unsigned long long* __restrict__ pats,
-                                                     int n_codes, u32x2* __restrict__ cd, int* __restrict__ flag) {
+                                                     i64 nruns, const unsigned long long* __restrict__ pats,
+                                                     int n_codes, unsigned* __restrict__ cd, int* __restrict__ flag) {
+    constexpr int W = RUN / 4;  // words of codes (and of deltas) per run
     __shared__ unsigned long long kset[1 << kDictLocalBits];
     __shared__ unsigned char kcode[1 << kDictLocalBits];
     constexpr unsigned mask = (1u << kDictLocalBits) - 1;
@@ -506,36 +508,49 @@ __global__ __launch_bounds__(256) void k_dict_encode(const VT* __restrict__ val,
     }
     __syncthreads();
     const i64 stride = static_cast<i64>(gridDim.x) * blockDim.x;
-    for (i64 q = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; q < nquads; q += stride) {
-        unsigned codes = 0;
+    for (i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; r < nruns; r += stride) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const unsigned long long p = value_bits(val[4 * q + k]);
-            unsigned h = dict_hash(p, kDictLocalBits);
-            int probe = 0;
-            while (kset[h] != p && kset[h] != kDictEmpty && probe <= static_cast<int>(mask)) {
-                h = (h + 1) & mask;
-                ++probe;
+        for (int w = 0; w < W; ++w) {
+            unsigned codes = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const unsigned long long p = value_bits(val[RUN * r + 4 * w + k]);
+                unsigned h = dict_hash(p, kDictLocalBits);
+                int probe = 0;
+                while (kset[h] != p && kset[h] != kDictEmpty && probe <= static_cast<int>(mask)) {
+                    h = (h + 1) & mask;
+                    ++probe;
+                }
+                if (kset[h] == p) codes |= static_cast<unsigned>(kcode[h]) << (8 * k);
+                else atomicOr(flag, 2);
             }
-            if (kset[h] == p) codes |= static_cast<unsigned>(kcode[h]) << (8 * k);
-            else atomicOr(flag, 2);
+            cd[2 * W * r + w] = codes;
+            cd[2 * W * r + W + w] = dlt4[W * r + w];
         }
-        cd[q] = u32x2{codes, dlt4[q]};
     }
 }
 
-void launch_dict_encode(const TiledLines& t, const unsigned long long* pats, int n_codes, u32x2* cd, int* flag,
+void launch_dict_encode(const TiledLines& t, const unsigned long long* pats, int n_codes, void* cd, int* flag,
                         hipStream_t s) {
-    const i64 nquads = t.slots / 4;
-    if (nquads <= 0) return;
-    const dim3 g(static_cast<unsigned>(std::max<i64>(1, std::min<i64>((nquads + 255) / 256, 8192)))), b(256);
+    const int run = t.dict_run;
+    if (run != 4 && run != 16) throw std::runtime_error("value dictionary runs hold 4 or 16 slots");
+    const i64 nruns = t.slots / run;
+    if (nruns <= 0) return;
+    const dim3 g(static_cast<unsigned>(std::max<i64>(1, std::min<i64>((nruns + 255) / 256, 8192)))), b(256);
     const unsigned* d4 = reinterpret_cast<const unsigned*>(t.dlt);
-    if (t.st == ST_F64)
-        hipLaunchKernelGGL(k_dict_encode<double>, g, b, 0, s, static_cast<const double*>(t.val), d4, nquads, pats,
-                           n_codes, cd, flag);
+    unsigned* out = static_cast<unsigned*>(cd);
+    if (t.st == ST_F64 && run == 16)
+        hipLaunchKernelGGL((k_dict_encode<double, 16>), g, b, 0, s, static_cast<const double*>(t.val), d4, nruns,
+                           pats, n_codes, out, flag);
+    else if (t.st == ST_F64)
+        hipLaunchKernelGGL((k_dict_encode<double, 4>), g, b, 0, s, static_cast<const double*>(t.val), d4, nruns,
+                           pats, n_codes, out, flag);
+    else if (t.st == ST_F32 && run == 16)
+        hipLaunchKernelGGL((k_dict_encode<float, 16>), g, b, 0, s, static_cast<const float*>(t.val), d4, nruns,
+                           pats, n_codes, out, flag);
     else if (t.st == ST_F32)
-        hipLaunchKernelGGL(k_dict_encode<float>, g, b, 0, s, static_cast<const float*>(t.val), d4, nquads, pats,
-                           n_codes, cd, flag);
+        hipLaunchKernelGGL((k_dict_encode<float, 4>), g, b, 0, s, static_cast<const float*>(t.val), d4, nruns,
+                           pats, n_codes, out, flag);
     else
         throw std::runtime_error("the value dictionary holds FP64 or FP32 values");
     IRLBA_HIP_LAUNCH_CHECK();
@@ -557,7 +572,7 @@ struct SpmvParams {
     const i64* chunk;
     const std::uint8_t* dlt;      // 8-bit delta indices (delta layout) ...
     const std::uint16_t* first;   // ... and the first index of each segment
-    const u32x2* cd;              // value dictionary: {4 codes, 4 deltas} per lane step ...
+    const void* cd;               // value dictionary: code stream in runs of DICT slots ...
     const void* dict;             // ... and the table (VT), n_dict entries
     int n_dict;
     i64 n_major, n_minor;
@@ -637,7 +652,26 @@ __device__ __forceinline__ int xcd_chunk(int b, int nb) {
     return (nb % kXcds == 0) ? (b % kXcds) * (nb / kXcds) + b / kXcds : b;
 }
 
-template <typename VT, typename AT, int LPL, int VEC, bool DELTA, bool DICT>
+// Decodes one run of 16 dictionary slots (four code words, four delta words)
+// and accumulates its terms; q enters as the index before the run's first
+// slot and leaves at its last.
+template <typename AT>
+__device__ __forceinline__ void dict_run16(const u32x4& c, const u32x4& d, unsigned& q, const AT* tab, const AT* xs,
+                                           AT& acc0, AT& acc1) {
+    const unsigned cw[4] = {c.x, c.y, c.z, c.w}, dw[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            q += (dw[j] >> (8 * k)) & 0xffu;
+            const AT v = tab[(cw[j] >> (8 * k)) & 0xffu];
+            if (j & 1) acc1 += v * xs[q];
+            else acc0 += v * xs[q];
+        }
+    }
+}
+
+template <typename VT, typename AT, int LPL, int VEC, bool DELTA, int DICT>
 __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
     extern __shared__ unsigned char smem_raw[];
     AT* xs = reinterpret_cast<AT*>(smem_raw);
@@ -654,15 +688,16 @@ __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
     constexpr int STRIDE = LPL * VEC;
     // Steps in flight per lane: the tile in LDS allows one 16-wave block per
     // CU, so up to 128 VGPRs cost no occupancy; FP64 values or sums fit four steps.
-    // Dictionary codes hold four registers per step whatever the precision.
-    constexpr int UNR = DICT ? 8 : ((sizeof(VT) == 8 || sizeof(AT) == 8) ? 4 : 8);
+    // Dictionary codes hold four registers per step (runs of 4; eight for
+    // runs of 16) whatever the precision.
+    constexpr int UNR = DICT == 16 ? 4 : DICT == 4 ? 8 : ((sizeof(VT) == 8 || sizeof(AT) == 8) ? 4 : 8);
     const Prologue& pro = p.pro;
     const bool single = p.n_tiles == 1;
     const double coef = (single && p.epi.axpy_src) ? *p.epi.axpy_coef : 0.0;
     // The value table sits after the input tile; the first tile's barrier
     // publishes it.
     [[maybe_unused]] AT* tab = xs + p.tile_len;
-    if constexpr (DICT) {
+    if constexpr (DICT != 0) {
         static_assert(DELTA, "the value dictionary is a form of the delta layout");
         const VT* dict = static_cast<const VT*>(p.dict);
         for (int j = threadIdx.x; j < p.n_dict; j += blockDim.x) tab[j] = static_cast<AT>(dict[j]);
@@ -740,15 +775,74 @@ __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
             // 1.50 -> 1.40 ms (with the lanes per line tuned for each form); a
             // form with each lane decoding its own two pairs (one scan of
             // packed pair sums, 16-bit delta loads) measured 1.77 / 1.46 ms.
-            constexpr bool PAIRS = sizeof(VT) == 8 && !DICT;
-            if constexpr (DICT) {
+            constexpr bool PAIRS = sizeof(VT) == 8 && DICT == 0;
+            if constexpr (DICT == 16) {
+                // Runs of 16 slots per lane step (a group step covers LPL runs,
+                // 32 bytes each); one cross-lane scan per 16 slots.  Segments
+                // hold whole runs.
+                constexpr int RS = LPL * 16;  // slots per group step
+                const u32x4* __restrict__ cp = static_cast<const u32x4*>(p.cd) + 2 * ((s0 >> 4) + lane_in);
+                int left = static_cast<int>(s1 - s0);
+                for (; left >= UNR * RS; left -= UNR * RS, cp += 2 * UNR * LPL) {
+                    u32x4 c[UNR], d[UNR];
+                    unsigned tot[UNR], inc[UNR];
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        c[u] = __builtin_nontemporal_load(cp + 2 * u * LPL);
+                        d[u] = __builtin_nontemporal_load(cp + 2 * u * LPL + 1);
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        tot[u] = __builtin_amdgcn_sad_u8(
+                            d[u].x, 0u, __builtin_amdgcn_sad_u8(
+                                            d[u].y, 0u, __builtin_amdgcn_sad_u8(
+                                                            d[u].z, 0u, __builtin_amdgcn_sad_u8(d[u].w, 0u, 0u))));
+                        inc[u] = group_scan_incl_dpp<LPL>(tot[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        unsigned q = carry + inc[u] - tot[u];
+                        carry += group_last<LPL>(inc[u]);
+                        dict_run16(c[u], d[u], q, tab, xs, acc0, acc1);
+                    }
+                }
+                if (left > 0) {
+                    u32x4 c[UNR], d[UNR];
+                    unsigned tot[UNR], inc[UNR];
+                    bool ok[UNR];
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        ok[u] = lane_in * 16 + u * RS < left;
+                        c[u] = u32x4{0, 0, 0, 0};
+                        d[u] = u32x4{0, 0, 0, 0};
+                        if (ok[u]) {
+                            c[u] = __builtin_nontemporal_load(cp + 2 * u * LPL);
+                            d[u] = __builtin_nontemporal_load(cp + 2 * u * LPL + 1);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        tot[u] = __builtin_amdgcn_sad_u8(
+                            d[u].x, 0u, __builtin_amdgcn_sad_u8(
+                                            d[u].y, 0u, __builtin_amdgcn_sad_u8(
+                                                            d[u].z, 0u, __builtin_amdgcn_sad_u8(d[u].w, 0u, 0u))));
+                        inc[u] = group_scan_incl_dpp<LPL>(tot[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        unsigned q = carry + inc[u] - tot[u];
+                        carry += group_last<LPL>(inc[u]);
+                        if (ok[u]) dict_run16(c[u], d[u], q, tab, xs, acc0, acc1);
+                    }
+                }
+            } else if constexpr (DICT == 4) {
                 // One 8-byte word per lane step: the codes and deltas of its
                 // four slots (lane l: slots 4l .. 4l+3 of the step).  Whole
                 // iterations inside the segment run without predicates (no
                 // compares, masks or branches: the product is bound by
                 // instruction issue, not by its 2 bytes per slot), then one
                 // predicated tail.
-                const u32x2* __restrict__ cp = p.cd + (s0 >> 2) + lane_in;
+                const u32x2* __restrict__ cp = static_cast<const u32x2*>(p.cd) + (s0 >> 2) + lane_in;
                 int left = static_cast<int>(s1 - s0);
                 for (; left >= UNR * STRIDE; left -= UNR * STRIDE, cp += UNR * LPL) {
                     unsigned cw[UNR], dw[UNR], tot[UNR], inc[UNR];
@@ -971,7 +1065,7 @@ static void spmv_launch_one(const TiledLines& tl, const Prologue& pro, const Epi
     p.chunk = tl.chunk;
     p.dlt = tl.dlt;
     p.first = tl.first;
-    p.cd = static_cast<const u32x2*>(tl.cd);
+    p.cd = tl.cd;
     p.dict = tl.dict;
     p.n_dict = tl.n_dict;
     p.n_major = tl.n_major;
@@ -987,14 +1081,16 @@ static void spmv_launch_one(const TiledLines& tl, const Prologue& pro, const Epi
     const size_t lds = (static_cast<size_t>(tl.tile_len) + (dict ? kDictMaxCodes : 0)) * sizeof(AT);
     if (ev0) IRLBA_HIP_CHECK(hipEventRecord(ev0, s));
     if constexpr (VEC == 4) {
-        if (dict)
-            hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, 4, true, true>), dim3(tl.B), dim3(1024), lds, s, p);
+        if (dict && tl.dict_run == 16)
+            hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, 4, true, 16>), dim3(tl.B), dim3(1024), lds, s, p);
+        else if (dict)
+            hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, 4, true, 4>), dim3(tl.B), dim3(1024), lds, s, p);
         else if (tl.delta)
-            hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, 4, true, false>), dim3(tl.B), dim3(1024), lds, s, p);
+            hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, 4, true, 0>), dim3(tl.B), dim3(1024), lds, s, p);
         else
-            hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, 4, false, false>), dim3(tl.B), dim3(1024), lds, s, p);
+            hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, 4, false, 0>), dim3(tl.B), dim3(1024), lds, s, p);
     } else {
-        hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, VEC, false, false>), dim3(tl.B), dim3(1024), lds, s, p);
+        hipLaunchKernelGGL((k_spmv_tiled<VT, AT, LPL, VEC, false, 0>), dim3(tl.B), dim3(1024), lds, s, p);
     }
     IRLBA_HIP_LAUNCH_CHECK();
     if (ev1) IRLBA_HIP_CHECK(hipEventRecord(ev1, s));

@@ -140,6 +140,7 @@ typedef struct irlba_hip_layout {
     int32_t delta;          /* 1: 8-bit delta indices */
     int32_t blocks;         /* workgroups of one product */
     int32_t dictionary;     /* entries of the value dictionary the products read, 0: the value stream */
+    int32_t dictionary_run; /* slots per code word group of that stream (4 or 16) */
 } irlba_hip_layout;
 int irlba_hip_sparse_layout(const irlba_hip_matrix* m, int adjoint, irlba_hip_layout* out);
 

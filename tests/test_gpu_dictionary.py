@@ -1,13 +1,15 @@
 """GPU parity of the value dictionary (irlba_hip_sparse_value_dictionary): the
 products stream one code byte per stored value and read the value from a
-table of the distinct stored values.
+table of the distinct stored values.  Each product term is the same number as
+with the value stream; the order of the sums depends on the code stream's runs.
 
-  * FP32 storage: each product term is the same number, added in the same
-    order as with the value stream, so at the same lanes per line the outputs
-    are equal bit for bit (and so is a whole solve).
-  * FP64 storage: the value stream loads lane pairs from both halves of a step
-    while the codes keep four consecutive slots per lane, so the same terms are
-    summed in another order: relative 1e-14 of the output.
+  * Runs of 4 slots per lane (orientations whose segments are not all whole
+    runs of 16) with FP32 storage: the terms are added in the value stream's
+    order, so at the same lanes per line the outputs are equal bit for bit (and
+    so is a whole solve).
+  * Runs of 16, or FP64 storage (whose value stream loads lane pairs from both
+    halves of a step): the same terms in another order, relative 1e-14 of the
+    output with FP64 accumulation, 2e-6 with FP32 accumulation.
   * Against the oracle's ParallelSparseMatrix restatement (include/irlba/Matrix/
     sparse.hpp:185-267) the value stream's tolerances hold: FP64 accumulation
     1e-12, FP32 accumulation 2e-6 of the output norm.
@@ -58,15 +60,19 @@ def test_dictionary_products_match_value_stream_and_oracle(storage, acc, nr, nc,
         n = H.value_dictionary(True)
     distinct = set(np.unique(vals.view(np.uint32 if f32 else np.uint64)).tolist()) | {0}
     assert n == len(distinct)
+    runs = []
     for adjoint in (False, True):
         lay = H.layout(adjoint)
         assert lay["dictionary"] == n and lay["delta"] == 1 and lay["lanes_per_line"] == lpl
+        assert lay["dictionary_run"] in (4, 16)
+        runs.append(lay["dictionary_run"])
     y1, z1 = H.multiply(v), H.adjoint_multiply(u)
-    if f32:
-        np.testing.assert_array_equal(y1, y0)
-        np.testing.assert_array_equal(z1, z0)
-    else:
-        assert _rel(y1, y0) < 1e-14 and _rel(z1, z0) < 1e-14
+    close = 1e-14 if acc == "f64" else 2e-6
+    for out, ref, run in ((y1, y0, runs[0]), (z1, z0, runs[1])):
+        if f32 and run == 4:
+            np.testing.assert_array_equal(out, ref)
+        else:
+            assert _rel(out, ref) < close
     OM = O.sparse(nr, nc, vals, idx, ptr, True, 1)
     tol = 1e-12 if acc == "f64" else 2e-6
     assert _rel(y1, OM.multiply(v)) < tol
@@ -96,14 +102,18 @@ def test_dictionary_signed_zero_and_negative_values():
         y0, z0 = H.multiply(v), H.adjoint_multiply(u)
         n = H.value_dictionary(True)
     assert n == len(set(np.unique(vals.view(np.uint32)).tolist()) | {0})
-    np.testing.assert_array_equal(H.multiply(v), y0)
-    np.testing.assert_array_equal(H.adjoint_multiply(u), z0)
+    for out, ref, adjoint in ((H.multiply(v), y0, False), (H.adjoint_multiply(u), z0, True)):
+        if H.layout(adjoint)["dictionary_run"] == 4:
+            np.testing.assert_array_equal(out, ref)
+        else:
+            assert _rel(out, ref) < 2e-6
 
 
 def test_dictionary_pca_equals_value_stream_and_oracle():
     # tests/src/pca.cpp:218-240: the centred+scaled chain over a dictionary-coded
-    # matrix.  FP32 storage: the whole solve equals the value stream's bit for bit.
-    nr, nc = 400, 6000
+    # matrix.  FP32 storage with runs of 4 in both orientations (short lines):
+    # the whole solve equals the value stream's bit for bit.
+    nr, nc = 400, 1000
     vals32, idx, ptr = O.gen_nb_sparse(nr, nc, 0.05, 23, f32=True)
     init = O.create_random_vector(nc, 8)
     opts = irlba.Options(extra_work=20, initial=init, convergence_tolerance=1e-8)
@@ -111,21 +121,32 @@ def test_dictionary_pca_equals_value_stream_and_oracle():
         H = irlba.HipSparseMatrix(nr, nc, vals32, idx, ptr, accumulate="f32")
         r0 = irlba.pca(H, True, True, 8, opts)
         H.value_dictionary(True)
+    assert H.layout(False)["dictionary_run"] == 4 and H.layout(True)["dictionary_run"] == 4
     r1 = irlba.pca(H, True, True, 8, opts)
     assert r1.iterations == r0.iterations
     np.testing.assert_array_equal(r1.variances, r0.variances)
     np.testing.assert_array_equal(r1.rotation, r0.rotation)
     np.testing.assert_array_equal(r1.scores, r0.scores)
-    # FP64 storage against the oracle with the same start vector.
+    # Longer rows (runs of 16 on A.v): FP64 storage against the oracle with the
+    # same start vector, and FP32 storage (FP32 products) within the mixed
+    # solve's 1e-5 of it.
+    nr, nc = 400, 6000
+    vals32, idx, ptr = O.gen_nb_sparse(nr, nc, 0.05, 23, f32=True)
     vals = vals32.astype(np.float64)
+    init = O.create_random_vector(nc, 8)
+    opts = irlba.Options(extra_work=20, initial=init, convergence_tolerance=1e-8)
     H64 = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr)
-    assert H64.value_dictionary(True) > 0
+    assert H64.value_dictionary(True) > 0 and H64.layout(False)["dictionary_run"] == 16
     res = irlba.pca(H64, True, True, 8, opts)
     sc, rot, var, conv, it = O.pca(O.sparse(nr, nc, vals, idx, ptr, True, 1), True, True, 8, extra_work=20,
                                    initial=init, convergence_tolerance=1e-8)
     assert res.converged and conv
     assert_equal_vectors(res.variances, var, 1e-8)
     assert subspace_angle(res.rotation, rot) < 1e-5
+    H32 = irlba.HipSparseMatrix(nr, nc, vals32, idx, ptr, accumulate="f32")
+    H32.value_dictionary(True)
+    r32 = irlba.pca(H32, True, True, 8, opts)
+    assert_equal_vectors(r32.variances, var, 1e-5)
 
 
 def test_dictionary_shared_by_decorators():
