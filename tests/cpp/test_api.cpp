@@ -216,6 +216,23 @@ int main() {
         bool iok = true;
         for (irlba::Index r = 0; r < nr; ++r) iok = iok && std::abs(y1[r] - y2[r]) <= 1e-12 * (1 + std::abs(y2[r]));
         check(iok, "HipSparseMatrix over integer values and 64-bit indices");
+        // Counts take few distinct values: the products can stream dictionary
+        // codes instead (same terms; FP64 sums in another order).
+        const int ncodes = isparse.use_value_dictionary();
+        irlba::VectorXd y3(nr);
+        isparse.new_workspace()->multiply(vv, y3);
+        bool dok = ncodes == 7;
+        for (irlba::Index r = 0; r < nr; ++r) dok = dok && std::abs(y3[r] - y1[r]) <= 1e-14 * (1 + std::abs(y1[r]));
+        dok = dok && isparse.use_value_dictionary(false) == 0;
+        check(dok, "HipSparseMatrix value dictionary: same products");
+        std::string dmsg;
+        try {
+            sparse.use_value_dictionary();
+        } catch (const std::exception& e) {
+            dmsg = e.what();
+        }
+        check(dmsg.find("more than 256 distinct values") != std::string::npos,
+              "HipSparseMatrix value dictionary refuses values with many distinct patterns");
         std::string emsg;
         std::vector<long> bad(idx64);
         bad[0] = static_cast<long>(nr) + (1L << 32);  // would wrap into range as int32
