@@ -128,6 +128,54 @@ def test_sparse_pca_30000x1e6_mixed_and_fp64():
     assert checked >= 45
 
 
+def test_sparse_pca_30000x1e6_value_dictionary():
+    # The same workload with the products streaming value-dictionary codes
+    # (irlba_hip_sparse_value_dictionary): the log1p NB values take a few dozen
+    # distinct values.  Products against the value stream's, then both solves
+    # checked with the property set above on the FP64 value-stream operator.
+    nr, nc, k, tol = 30000, 1000000, 50, 1e-5
+    init = np.random.default_rng(1234).standard_normal(nc)
+    opts = irlba.Options(extra_work=20, initial=init, convergence_tolerance=tol)
+    x = torch.randn(nc, dtype=torch.float64, device="cuda").cpu().numpy()
+    u = torch.randn(nr, dtype=torch.float64, device="cuda").cpu().numpy()
+    results = {}
+    for storage, close in (("f32", 2e-6), ("f64", 1e-14)):
+        _say(f"sparse pca 30000x1e6 with the value dictionary: {storage} storage")
+        M = irlba.generate_sparse(nr, nc, 0.03, 42, storage=storage, accumulate=storage)
+        y0, z0 = M.multiply(x), M.adjoint_multiply(u)
+        n = M.value_dictionary(True)
+        y1, z1 = M.multiply(x), M.adjoint_multiply(u)
+        ry = np.linalg.norm(y1 - y0) / np.linalg.norm(y0)
+        rz = np.linalg.norm(z1 - z0) / np.linalg.norm(z0)
+        _say(f"  {n} codes, runs {M.layout(False)['dictionary_run']}/{M.layout(True)['dictionary_run']}; "
+             f"products vs the value stream: {ry:.2e} / {rz:.2e}")
+        assert 2 <= n <= 256 and ry < close and rz < close
+        results[storage] = irlba.pca(M, True, True, k, opts)
+        _say(f"  converged={results[storage].converged}, {results[storage].iterations} restarts")
+        if storage == "f64":
+            M.value_dictionary(False)
+            A = M
+        del M
+    mixed, f64 = results["f32"], results["f64"]
+    assert mixed.converged and f64.converged
+    c, s = irlba.column_stats(A)
+    Ac = irlba.ScaledMatrix(irlba.CenteredMatrix(A, c), s, column=True, divide=True)
+    sm, fm, am = _triplet_residuals(Ac, mixed.scores, mixed.rotation, mixed.variances, nr, k)
+    sd, fd, ad = _triplet_residuals(Ac, f64.scores, f64.rotation, f64.variances, nr, k)
+    _say(f"  residual column norms / s_1: mixed fwd {fm.max() / sd[0]:.2e} adj {am.max() / sd[0]:.2e}; "
+         f"fp64 fwd {fd.max() / sd[0]:.2e} adj {ad.max() / sd[0]:.2e}")
+    for res in (mixed, f64):
+        R = res.rotation
+        assert np.abs(R.T @ R - np.eye(k)).max() < 1e-10
+        assert np.all(np.abs(res.scores.sum(axis=0)) < 1e-6 * np.sqrt(res.variances * (nr - 1)))
+    # Tolerances as for the value stream's runs above.
+    assert ad.max() <= tol * sd[0] and am.max() <= 2 * tol * sd[0]
+    assert fd.max() / sd[0] < 1e-9 and fm.max() / sd[0] < 1e-5
+    rel = np.max(np.abs(sm - sd) / sd)
+    _say(f"  max singular value rel. error mixed vs fp64 (both coded): {rel:.3e}")
+    assert rel < 1e-5
+
+
 # ---------------------------------------------------------------- dense 200000 x 2000
 def _gram_fp64(X):
     """X' X in FP64 on the device, by 20000-row panels."""
