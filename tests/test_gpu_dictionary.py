@@ -196,3 +196,36 @@ def test_dictionary_refusals_leave_the_value_stream():
         irlba.HipSparseMatrix(2000, 2000, svals, sidx, sptr).value_dictionary(True)
     with pytest.raises(irlba.HipError, match="not a sparse matrix handle"):
         irlba.HipDenseMatrix(np.ones((30, 20))).value_dictionary(True)
+
+
+def test_dictionary_csr_input_vs_oracle():
+    # Row-major (CSR) input: the same two tiled orientations, built from rows.
+    nr, nc = 3000, 20000
+    vals, idx, ptr = O.gen_nb_sparse(nc, nr, 0.05, 37, f32=False)  # CSC of the transpose = CSR of the matrix
+    H = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, column_major=False)
+    OM = O.sparse(nr, nc, vals, idx, ptr, False, 1)
+    v = O.create_random_vector(nc, 11)
+    u = O.create_random_vector(nr, 12)
+    assert H.value_dictionary(True) > 0
+    assert _rel(H.multiply(v), OM.multiply(v)) < 1e-12
+    assert _rel(H.adjoint_multiply(u), OM.multiply(u, adjoint=True)) < 1e-12
+
+
+def test_dictionary_svd_backward_matches_value_stream():
+    # The backward's forward solve streams the codes; its CG block products read
+    # the value stream.  Same gradient factors as without the dictionary.
+    nr, nc, k = 2000, 6000, 6
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.05, 41, f32=False)
+    H = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr)
+    opts = irlba.Options(extra_work=20, convergence_tolerance=1e-12, initial=O.create_random_vector(nc, 2))
+    rng = np.random.default_rng(5)
+    w, GU, GV = rng.standard_normal(k), rng.standard_normal((nr, k)), rng.standard_normal((nc, k))
+    out = []
+    for on in (False, True):
+        H.value_dictionary(on)
+        res = irlba.compute(H, k, opts)
+        bw = irlba.svd_backward(H, res.U, res.D, res.V, w, GU, GV, complement="exact")
+        out.append((res, bw.dense()))
+    (r0, g0), (r1, g1) = out
+    assert_equal_vectors(r1.D, r0.D, 1e-12)
+    assert np.abs(g1 - g0).max() <= 1e-7 * np.abs(g0).max()
