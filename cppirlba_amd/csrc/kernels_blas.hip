@@ -32,12 +32,17 @@ static inline i64 cgs_chunk(i64 L, int G) { return (L + G - 1) / G; }
 
 // Long vectors: 8192-row chunks, each block looping over its chunk (fewer,
 // longer-lived blocks and fewer partials: 90 us against 100 us for one-pass
-// 1024-row blocks on a 1e6 x 67 basis), at most 1024 blocks.  Short ones:
-// 256-row chunks (U = 1) so that enough blocks share the work.
+// 1024-row blocks on a 1e6 x 67 basis), at most 1024 blocks.  Their count is
+// rounded up to a multiple of the 8 XCDs: workgroups go to the XCDs round-robin
+// by linear id (x + y * gridDim.x), so every column batch (y) of a row chunk (x)
+// then runs on one XCD and the chunk of f comes from that XCD's L2 after the
+// first batch instead of from the MALL nine times (96.4 -> 90.6 us on a
+// 1e6 x 67 basis).  Short ones: 256-row chunks (U = 1) so that enough blocks
+// share the work.
 constexpr i64 kDotLong = 262144;
 int cgs_dot_blocks(i64 L) {
-    const i64 per = L >= kDotLong ? 8192 : 256;
-    return static_cast<int>(std::max<i64>(1, std::min<i64>(1024, (L + per - 1) / per)));
+    if (L >= kDotLong) return static_cast<int>(std::min<i64>(1024, ((L + 8191) / 8192 + 7) / 8 * 8));
+    return static_cast<int>(std::max<i64>(1, std::min<i64>(1024, (L + 255) / 256)));
 }
 
 template <int C, int U>
