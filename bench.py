@@ -121,6 +121,8 @@ def spmv_mode(args):
     storage = args.storage
     mat = irlba.generate_sparse(args.rows, args.cols, args.density, args.seed, storage=storage,
                                 accumulate="f64" if storage == "f64" else "f32")
+    if args.dictionary:
+        mat.value_dictionary(True)
     ctx.synchronize()
     L = _lib.lib()
     x = torch.randn(args.cols, dtype=torch.float64, device="cuda")
@@ -136,7 +138,7 @@ def spmv_mode(args):
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"mode": "spmv", "storage": storage, "reps": args.reps,
+    print(json.dumps({"mode": "spmv", "storage": storage, "dictionary": bool(args.dictionary), "reps": args.reps,
                       "forward_ms": st["forward_kernel_ms"] / max(1, st["forward_timed"]),
                       "adjoint_ms": st["adjoint_kernel_ms"] / max(1, st["adjoint_timed"]),
                       "forward_bytes": st["forward_bytes"], "adjoint_bytes": st["adjoint_bytes"],
@@ -361,6 +363,8 @@ def main():
     ap.add_argument("--dense-cols", type=int, default=2000)
     ap.add_argument("--dense-reps", type=int, default=1)
     ap.add_argument("--dense-storages", default="bf16,f32,f64")
+    ap.add_argument("--dictionary", action="store_true",
+                    help="--mode spmv: stream the value-dictionary codes instead of the values")
     ap.add_argument("--skip-dictionary", action="store_true",
                     help="skip the value-dictionary solves (codes in place of the stored values)")
     ap.add_argument("--skip-backward-exact-sparse", action="store_true",
@@ -538,6 +542,18 @@ def main():
             f"{res_d.iterations} restarts")
         dict_out = {"distinct_values": n_codes, "build_s": round(build_dict_s, 3),
                     "mixed": _dict_leg(mixed, res_d, dt_d, st_d, nr, nc)}
+        prof = os.path.join(ROOT, "profiles", "spmv_traffic.json")
+        if os.path.exists(prof):
+            with open(prof) as f:
+                tr = json.load(f)
+            if "dictionary_forward_hbm_bytes_per_launch" in tr:
+                # Measured HBM bytes of one coded A.v (committed counter profile), over this run's launch time.
+                t_b = tr["dictionary_forward_hbm_bytes_per_launch"]
+                dict_out["mixed"]["forward_traffic"] = t_b
+                dict_out["mixed"]["forward_traffic_frac"] = round(
+                    t_b / (dict_out["mixed"]["forward_ms"] * 1e6) / HBM_PEAK_GBS, 4)
+                dict_out["traffic_source"] = ("profiles/spmv_traffic.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
+                                              "runs of bench.py --mode spmv --dictionary); not measured in this run")
         mixed.value_dictionary(False)
 
     if not args.skip_fp64:
