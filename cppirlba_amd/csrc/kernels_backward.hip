@@ -68,7 +68,9 @@ __global__ __launch_bounds__(256) void k_sddmm(const std::int32_t* __restrict__ 
                                                int* __restrict__ flag) {
     typedef typename Piece<FT>::type PT;
     constexpr int VN = Piece<FT>::n;
-    constexpr int G = kWave / LPE;  // entries in flight per wavefront
+    constexpr int G = kWave / LPE;  // entries per wavefront step
+    constexpr int CH = LPE >= 16 ? 1 : 16 / LPE;
+    constexpr int NB = CH * LPE < 4 ? CH * LPE : 4;
     const int lane = threadIdx.x & 63, q = lane % LPE, g = lane / LPE;
     const i64 wave = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const i64 nwaves = (static_cast<i64>(gridDim.x) * blockDim.x) >> 6;
@@ -81,21 +83,46 @@ __global__ __launch_bounds__(256) void k_sddmm(const std::int32_t* __restrict__ 
             if (lane == 0) atomicOr(flag, 1);
             continue;
         }
-        // Every lane runs the same trip count so the group shuffles stay converged.
-        for (i64 base = e0; base < e1; base += G) {
-            const i64 e = base + g;
-            double acc = 0;
-            const i64 row = e < e1 ? idx[e] : 0;
-            const bool in = row >= 0 && row < n_other;
-            if (e < e1 && !in && q == 0) atomicOr(flag, 2);
-            if (e < e1 && in) {
-                // A normal (cached) load: each factor row is shared by every entry of its row.
-                const PT x = *reinterpret_cast<const PT*>(entry_rm + row * Rp + q * VN);
+        // Passes of CH chunks of 64 entries: each lane loads one entry's index
+        // per chunk (coalesced, all chunks at once), the indices reach their
+        // groups by shuffles, and the gathers follow in batches of NB, so the
+        // HBM latency of the indices is paid once per pass, not once per G
+        // entries.  Every lane runs the same trip count, so the group shuffles
+        // stay converged.  Each entry's dot and group sum are formed as in a
+        // walk of G entries at a time.
+        for (i64 ps = e0; ps < e1; ps += 64 * CH) {
+            int r[CH];
 #pragma unroll
-                for (int v = 0; v < VN; ++v) acc += static_cast<double>(x[v]) * static_cast<double>(yl[v]);
+            for (int ch = 0; ch < CH; ++ch) {
+                const i64 e = ps + 64 * ch + lane;
+                r[ch] = e < e1 ? idx[e] : 0;
+                if (e < e1 && !(r[ch] >= 0 && r[ch] < n_other)) atomicOr(flag, 2);
             }
-            acc = group_sum<LPE>(acc);
-            if (q == 0 && e < e1) out[e] = static_cast<FT>(acc);
+#pragma unroll
+            for (int n0 = 0; n0 < CH * LPE; n0 += NB) {
+                double acc[NB];
+#pragma unroll
+                for (int i = 0; i < NB; ++i) {
+                    const int ch = (n0 + i) / LPE, j = g + G * ((n0 + i) % LPE);
+                    const i64 e = ps + 64 * ch + j;
+                    const int row = __shfl(r[ch], j, kWave);
+                    const bool ok = e < e1 && row >= 0 && row < n_other;
+                    // A normal (cached) load: each factor row is shared by every entry of its row.
+                    // Absent entries read a row that exists (the first of either factor).
+                    const FT* src = ok ? entry_rm + static_cast<i64>(row) * Rp : (n_other > 0 ? entry_rm : line_rm);
+                    const PT x = *reinterpret_cast<const PT*>(src + q * VN);
+                    double a = 0;
+#pragma unroll
+                    for (int v = 0; v < VN; ++v) a += static_cast<double>(x[v]) * static_cast<double>(yl[v]);
+                    acc[i] = ok ? a : 0.0;
+                }
+#pragma unroll
+                for (int i = 0; i < NB; ++i) {
+                    const double a = group_sum<LPE>(acc[i]);
+                    const i64 e = ps + 64 * ((n0 + i) / LPE) + g + G * ((n0 + i) % LPE);
+                    if (q == 0 && e < e1) out[e] = static_cast<FT>(a);
+                }
+            }
         }
     }
 }

@@ -42,6 +42,14 @@ __global__ __launch_bounds__(256) void k_spmm_lines(const i64* __restrict__ ptr,
     constexpr int VN = Piece16<FT>::n;
     constexpr int G = kWave / LPE;
     const int lane = threadIdx.x & 63, q = lane % LPE, g = lane / LPE;
+    // A pass covers CH chunks of 64 entries of a segment: each lane loads one
+    // entry's value and index per chunk (coalesced, all chunks at once), the
+    // entries reach their groups by shuffles, and the gathers follow in
+    // batches of NB, so the HBM latency of the values and indices is paid once
+    // per pass instead of once per few entries of each group.  Group g takes
+    // entries g, g + G, g + 2G, ... of the segment, in that order.
+    constexpr int CH = LPE >= 16 ? 1 : 16 / LPE;
+    constexpr int NB = CH * LPE < 8 ? CH * LPE : 8;
     const i64 wave = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const i64 nwaves = (static_cast<i64>(gridDim.x) * blockDim.x) >> 6;
     for (i64 m = wave; m < n_major; m += nwaves) {
@@ -51,34 +59,38 @@ __global__ __launch_bounds__(256) void k_spmm_lines(const i64* __restrict__ ptr,
         for (int t = 0; t < n_tiles; ++t) {
             const i64 seg = static_cast<i64>(t) * n_major + m;
             const i64 s0 = ptr[seg], s1 = ptr[seg + 1];
-            const i64 base = static_cast<i64>(t) * tile_len;
-            // Four entries per group in flight: their value/index loads, then
-            // their gathers, are issued back to back.
-            i64 s = s0 + g;
-            for (; s + 3 * G < s1; s += 4 * G) {
-                double a[4];
-                i64 r[4];
+            const FT* __restrict__ Xt = X + static_cast<i64>(t) * tile_len * kp + q * VN;
+            for (i64 ps = s0; ps < s1; ps += 64 * CH) {
+                VT a[CH];
+                int r[CH];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    a[u] = to_double_any(__builtin_nontemporal_load(val + s + u * G));
-                    r[u] = base + idx[s + u * G];
+                for (int c = 0; c < CH; ++c) {
+                    const i64 e = ps + 64 * c + lane;
+                    a[c] = VT(0);
+                    r[c] = 0;  // past the segment: value 0 at row `base`, a valid row
+                    if (e < s1) {
+                        a[c] = __builtin_nontemporal_load(val + e);
+                        r[c] = idx[e];
+                    }
                 }
-                PT x[4];
+                // Entry n = c LPE + u of the pass is entry g + G u of chunk c;
+                // batches of NB gathers, each batch's loads issued together.
 #pragma unroll
-                for (int u = 0; u < 4; ++u)  // padding slots (value 0) gather row `base`, a valid row
-                    x[u] = *reinterpret_cast<const PT*>(X + r[u] * kp + q * VN);
+                for (int n0 = 0; n0 < CH * LPE; n0 += NB) {
+                    double aj[NB];
+                    PT x[NB];
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
+                    for (int i = 0; i < NB; ++i) {
+                        const int c = (n0 + i) / LPE, j = g + G * ((n0 + i) % LPE);
+                        aj[i] = to_double_any(static_cast<VT>(__shfl(a[c], j, kWave)));
+                        const int rj = __shfl(r[c], j, kWave);
+                        x[i] = *reinterpret_cast<const PT*>(Xt + static_cast<i64>(rj) * kp);
+                    }
 #pragma unroll
-                    for (int v = 0; v < VN; ++v) acc[v] += a[u] * static_cast<double>(x[u][v]);
-            }
-            for (; s < s1; s += G) {
-                const double a = to_double_any(__builtin_nontemporal_load(val + s));
-                if (a == 0) continue;  // padding slots (and stored zeros) contribute nothing
-                const i64 r = base + idx[s];
-                const PT x = *reinterpret_cast<const PT*>(X + r * kp + q * VN);
+                    for (int i = 0; i < NB; ++i)
 #pragma unroll
-                for (int v = 0; v < VN; ++v) acc[v] += a * static_cast<double>(x[v]);
+                        for (int v = 0; v < VN; ++v) acc[v] += aj[i] * static_cast<double>(x[i][v]);
+                }
             }
         }
 #pragma unroll

@@ -71,19 +71,25 @@ def test_forced_delta_matches_numpy(monkeypatch, tile_bytes, storage):
     np.testing.assert_array_equal(h.realize(), A if storage == "f64" else A.astype(np.float32).astype(np.float64))
 
 
+@pytest.mark.parametrize("storage,acc", [("f64", "f64"), ("f32", "f32"), ("f32", "f64")])
 @pytest.mark.parametrize("lpl", ["4", "8", "16", "32", "64"])
-def test_delta_every_group_width(monkeypatch, lpl):
-    # The lane scan is written per group width (DPP row shifts and broadcasts).
+def test_delta_every_group_width(monkeypatch, lpl, storage, acc):
+    # The lane scan and the broadcast of each step's group total are written
+    # per group width (DPP row shifts, quad permutes, half-row mirrors, row
+    # broadcasts, lane reads); FP64 values take the lane-pair loads.
     A = _framed(1800, 1400, int(lpl))
     vals, idx, ptr = _csc(A)
     monkeypatch.setenv("IRLBA_HIP_DELTA", "2")
     monkeypatch.setenv("IRLBA_HIP_LPL", lpl)
-    h = irlba.HipSparseMatrix(1800, 1400, vals, idx, ptr, storage="f64")
+    h = irlba.HipSparseMatrix(1800, 1400, vals, idx, ptr, storage=storage, accumulate=acc)
     assert h.layout(False)["lanes_per_line"] == int(lpl) and h.layout(False)["delta"] == 1
+    if storage == "f32":
+        A = A.astype(np.float32).astype(np.float64)  # the stored matrix
+    tol = 1e-12 if acc == "f64" else 2e-6
     rng = np.random.default_rng(3)
     x, u = rng.standard_normal(1400), rng.standard_normal(1800)
-    assert _rel(h.multiply(x), A @ x) < 1e-12
-    assert _rel(h.adjoint_multiply(u), A.T @ u) < 1e-12
+    assert _rel(h.multiply(x), A @ x) < tol
+    assert _rel(h.adjoint_multiply(u), A.T @ u) < tol
 
 
 def test_layouts_agree_and_heuristic(monkeypatch):

@@ -86,6 +86,30 @@ def test_dictionary_products_match_value_stream_and_oracle(storage, acc, nr, nc,
     assert H.value_dictionary(True) == n
 
 
+@pytest.mark.parametrize("storage,acc", [("f32", "f32"), ("f64", "f64")])
+@pytest.mark.parametrize("lpl", [8, 32, 64])
+def test_dictionary_other_group_widths(storage, acc, lpl):
+    # The widths the test above leaves out: the step totals of the code
+    # stream are broadcast per width (half-row mirror, lane reads).
+    f32 = storage == "f32"
+    nr, nc = 1500, 40000
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.05, 55 + lpl, f32=f32)
+    with fixed_lanes(lpl):
+        H = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, storage=storage, accumulate=acc)
+        v = O.create_random_vector(nc, 13)
+        u = O.create_random_vector(nr, 14)
+        y0, z0 = H.multiply(v), H.adjoint_multiply(u)
+        H.value_dictionary(True)
+    assert H.layout(False)["lanes_per_line"] == lpl and H.layout(False)["dictionary"] > 0
+    close = 1e-14 if acc == "f64" else 2e-6
+    y1, z1 = H.multiply(v), H.adjoint_multiply(u)
+    assert _rel(y1, y0) < close and _rel(z1, z0) < close
+    OM = O.sparse(nr, nc, vals, idx, ptr, True, 1)
+    tol = 1e-12 if acc == "f64" else 2e-6
+    assert _rel(y1, OM.multiply(v)) < tol
+    assert _rel(z1, OM.multiply(u, adjoint=True)) < tol
+
+
 def test_dictionary_signed_zero_and_negative_values():
     # -0.0 and +0.0 are distinct bit patterns (two codes); negative values sort first.
     rng = np.random.default_rng(7)
