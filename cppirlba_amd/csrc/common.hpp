@@ -323,10 +323,11 @@ void launch_fill(double* out, double v, i64 n, hipStream_t s);
 
 // Multi-vector products on the tiled layout (kernels_spmm.hip).  X packed
 // row-major with stride spmm_padded_k(k, x_f32) (FP32 when x_f32), Y row-major
-// FP64 with the same stride.
+// FP64 with the same stride.  A stage call fills packed columns [c0, c0 + kw)
+// from k <= kw input columns (zeros after them).
 int spmm_padded_k(int k, bool x_f32, int* lanes_per_entry = nullptr);
-void launch_spmm_stage(const double* X, i64 ld, i64 L, int k, int kp, const double* xscale, bool x_f32, void* out,
-                       hipStream_t s);
+void launch_spmm_stage(const double* X, i64 ld, i64 L, int k, int c0, int kw, int kp, const double* xscale,
+                       bool x_f32, void* out, hipStream_t s);
 void launch_spmm_lines(const TiledLines& t, const void* X, int k, bool x_f32, double* Y, hipStream_t s);
 void launch_spmm_finish(const double* Yrm, i64 n, int k, int kp, const double* oscale, int n_terms,
                         const double* t0, const double* t1, const double* gam, double* Y, i64 ldy, hipStream_t s);

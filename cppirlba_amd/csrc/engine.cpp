@@ -371,24 +371,49 @@ void apply_handle(const irlba_hip_matrix& m, bool adjoint, const double* x, doub
     m.base->product(adjoint, pro, epi, e0, e1);
 }
 
-// Y[:, :k] = M X[:, :k] (or M' X) for k column vectors (column-major, leading
-// dimensions ldx, ldy).  Sparse and dense handles stream the matrix once for all
-// k (kernels_spmm.hip; dense up to 32 vectors per pass); otherwise k single
-// products.
-void apply_handle_multi(const irlba_hip_matrix& m, bool adjoint, const double* X, i64 ldx, double* Y, i64 ldy,
-                        int k) {
+// One input block of a multi-vector product: k columns of X (leading dimension
+// ldx) into k columns of Y (ldy).
+struct BlockIO {
+    const double* X;
+    i64 ldx;
+    double* Y;
+    i64 ldy;
+};
+
+// Whether nio blocks of k vectors each fit one pass of the block kernels
+// (sparse: up to 64 lanes of 16 bytes per entry; dense: 32 packed columns).
+bool blocks_fit_one_pass(const irlba_hip_matrix& m, bool adjoint, int k, int nio) {
     auto* sb = dynamic_cast<SparseBase*>(m.base.get());
     auto* db = dynamic_cast<DenseBase*>(m.base.get());
-    const bool dense_x_f32 = db && db->d.st != ST_F64;
-    if (k == 1 || (!sb && !db) || (db && spmm_padded_k(k, dense_x_f32) > 32)) {
-        for (int v = 0; v < k; ++v) apply_handle(m, adjoint, X + v * ldx, Y + v * ldy, nullptr, nullptr);
+    const int K = k * nio;
+    if (K <= 1 || (!sb && !db)) return false;
+    if (db) return spmm_padded_k(K, db->d.st != ST_F64) <= 32;
+    const bool x_f32 = (adjoint ? sb->by_col : sb->by_row).st == ST_F32 && !sb->acc64;
+    return K * (x_f32 ? 4 : 8) <= kWave * 16;
+}
+
+// Y_b[:, :k] = M X_b[:, :k] (or M' X_b) for nio blocks of k column vectors at
+// once.  Sparse and dense handles stream the matrix once for all nio k vectors
+// (kernels_spmm.hip), the blocks packed side by side; blocks that do not fit
+// one pass run one at a time, and a single vector as a single product.
+void apply_handle_blocks(const irlba_hip_matrix& m, bool adjoint, const BlockIO* io, int nio, int k) {
+    if (nio > 1 && !blocks_fit_one_pass(m, adjoint, k, nio)) {
+        for (int b = 0; b < nio; ++b) apply_handle_blocks(m, adjoint, io + b, 1, k);
         return;
     }
+    if (nio == 1 && !blocks_fit_one_pass(m, adjoint, k, 1)) {
+        for (int v = 0; v < k; ++v)
+            apply_handle(m, adjoint, io[0].X + v * io[0].ldx, io[0].Y + v * io[0].ldy, nullptr, nullptr);
+        return;
+    }
+    auto* sb = dynamic_cast<SparseBase*>(m.base.get());
+    auto* db = dynamic_cast<DenseBase*>(m.base.get());
     irlba_hip_context* ctx = m.ctx;
     hipStream_t s = ctx->stream;
     const Affine& a = m.aff;
-    const bool x_f32 = sb ? ((adjoint ? sb->by_col : sb->by_row).st == ST_F32 && !sb->acc64) : dense_x_f32;
-    const int kp = spmm_padded_k(k, x_f32);
+    const int K = k * nio;
+    const bool x_f32 = sb ? ((adjoint ? sb->by_col : sb->by_row).st == ST_F32 && !sb->acc64) : db->d.st != ST_F64;
+    const int kp = spmm_padded_k(K, x_f32);
     const i64 L_in = adjoint ? m.base->nr : m.base->nc, L_out = adjoint ? m.base->nc : m.base->nr;
     const i64 dscratch = db ? dense_block_scratch(db->d, kp) : 0;
     const double* xscale = adjoint ? a.dr.get() : a.dc.get();
@@ -405,7 +430,7 @@ void apply_handle_multi(const irlba_hip_matrix& m, bool adjoint, const double* X
         p.clear();
         p.push_back(c.take<char>(static_cast<i64>(x_f32 ? 4 : 8) * kp * L_in));       // packed X
         p.push_back(c.take<double>(static_cast<i64>(kp) * L_out));                     // packed Y
-        p.push_back(c.take<double>(2 * k));                                            // gammas
+        p.push_back(c.take<double>(2 * static_cast<i64>(K)));                          // gammas
         p.push_back(c.take<double>(static_cast<i64>(cgs_dot_blocks(L_in)) * k));      // dot partials
         p.push_back(need_ones ? c.take<double>(L_in) : nullptr);                       // ones
         p.push_back(db ? c.take<double>(dscratch) : nullptr);                          // dense partials
@@ -420,20 +445,33 @@ void apply_handle_multi(const irlba_hip_matrix& m, bool adjoint, const double* X
     double* partials = static_cast<double*>(wp[3]);
     double* ones = static_cast<double*>(wp[4]);
     if (need_ones) launch_fill(ones, 1.0, L_in, s);
-    for (int j = 0; j < a.n_terms; ++j) {
-        // gamma_j[v] = tdot_j . X[:, v] on the unscaled input (the single product's prologue).
-        launch_cgs_dots(X, ldx, k, tdot[j] ? tdot[j] : ones, L_in, partials, s);
-        launch_cgs_reduce(partials, L_in, k, gam + j * k, s);
+    for (int b = 0; b < nio; ++b) {
+        for (int j = 0; j < a.n_terms; ++j) {
+            // gamma_j[v] = tdot_j . X[:, v] on the unscaled input (the single product's prologue).
+            launch_cgs_dots(io[b].X, io[b].ldx, k, tdot[j] ? tdot[j] : ones, L_in, partials, s);
+            launch_cgs_reduce(partials, L_in, k, gam + (2 * b + j) * k, s);
+        }
+        // The last block also zeroes the padding columns.
+        launch_spmm_stage(io[b].X, io[b].ldx, L_in, k, b * k, b + 1 < nio ? k : kp - b * k, kp, xscale, x_f32, wp[0],
+                          s);
     }
-    launch_spmm_stage(X, ldx, L_in, k, kp, xscale, x_f32, wp[0], s);
     if (sb)
-        launch_spmm_lines(sb->indexed(adjoint), wp[0], k, x_f32, static_cast<double*>(wp[1]), s);
+        launch_spmm_lines(sb->indexed(adjoint), wp[0], K, x_f32, static_cast<double*>(wp[1]), s);
     else
         launch_dense_block(db->d, adjoint, kp, x_f32, wp[0], static_cast<double*>(wp[1]), static_cast<double*>(wp[5]),
                            dscratch, s);
-    launch_spmm_finish(static_cast<const double*>(wp[1]), L_out, k, kp, oscale, a.n_terms, tvec[0], tvec[1], gam, Y,
-                       ldy, s);
-    if (adjoint) ctx->stats.adjoint_products += k; else ctx->stats.forward_products += k;
+    for (int b = 0; b < nio; ++b)
+        launch_spmm_finish(static_cast<const double*>(wp[1]) + b * k, L_out, k, kp, oscale, a.n_terms, tvec[0],
+                           tvec[1], gam + 2 * b * k, io[b].Y, io[b].ldy, s);
+    if (adjoint) ctx->stats.adjoint_products += K; else ctx->stats.forward_products += K;
+}
+
+// Y[:, :k] = M X[:, :k] (or M' X) for k column vectors (column-major, leading
+// dimensions ldx, ldy).
+void apply_handle_multi(const irlba_hip_matrix& m, bool adjoint, const double* X, i64 ldx, double* Y, i64 ldy,
+                        int k) {
+    const BlockIO io{X, ldx, Y, ldy};
+    apply_handle_blocks(m, adjoint, &io, 1, k);
 }
 
 // ------------------------------------------------------------------ sparse construction
@@ -1430,14 +1468,17 @@ struct LockstepCg {
     hipStream_t s;
     int k, ldy;
 
+    // ap holds B in; ap = basis (basis' ap) - ap + sig2 in = sig2 in - P B in.
+    void project(CgSet& c, const double* in) {
+        launch_gram(c.basis, c.L, c.ap, c.L, k, c.L, c.partials, c.proj, ldy, s);
+        launch_rotate(c.basis, c.L, k, c.proj, ldy, k, c.uc, c.L, c.L, s);
+        launch_cg_op_finish(c.ap, c.uc, in, c.sig2, c.L, k, s);
+    }
     // out = sig2 p - P B p for every column, into set.ap.
     void op(CgSet& c, const double* in) {
         apply_handle_multi(*m, c.left, in, c.L, c.tmp, c.Lo, k);      // A' in (left) or A in (right)
         apply_handle_multi(*m, !c.left, c.tmp, c.Lo, c.ap, c.L, k);   // then A (A' in) or A' (A in)
-        // ap = basis (basis' ap) - ap + sig2 in
-        launch_gram(c.basis, c.L, c.ap, c.L, k, c.L, c.partials, c.proj, ldy, s);
-        launch_rotate(c.basis, c.L, k, c.proj, ldy, k, c.uc, c.L, c.L, s);
-        launch_cg_op_finish(c.ap, c.uc, in, c.sig2, c.L, k, s);
+        project(c, in);
     }
     void start(CgSet& c) {
         // x0 = b / sig2 (the rank-k answer), r0 = b - op(x0), p0 = r0.
@@ -1451,6 +1492,10 @@ struct LockstepCg {
     }
     void step(CgSet& c) {
         op(c, c.p);
+        update(c);
+    }
+    // The CG update once ap = op(p).
+    void update(CgSet& c) {
         launch_coldots(c.p, c.L, c.ap, c.L, k, c.L, c.partials, c.pap, s);
         // alpha = rr / pap (0 where pap <= 0); x += alpha p; r -= alpha ap; rr_new = |r|^2
         launch_cg_update_xr(c.x, c.r, c.p, c.ap, c.rr, c.pap, c.L, k, c.partials, c.rr_new, s);
@@ -1622,9 +1667,30 @@ int run_svd_backward(const irlba_hip_matrix* m, i64 k, const double* U, const do
         };
         int slot = 0;
         queue_check(slot);
+        // The right set runs half an iteration ahead of the left one, so that
+        // each pass over the matrix serves both: A' [p_left | A p_right] gives
+        // the left set's first product and the right set's second, then
+        // A [A' p_left | p_right] the left set's second and the right set's
+        // first of its next iteration.  Two products of 2k vectors per
+        // iteration instead of four of k (on the 50000 x 20000 1% matrix a
+        // 32-vector product takes ~1.3 times a 16-vector one).  Where 2k
+        // vectors do not fit one pass the sets step one after the other.
+        const bool paired = blocks_fit_one_pass(*m, true, ki, 2) && blocks_fit_one_pass(*m, false, ki, 2);
+        if (paired) apply_handle_multi(*m, false, set[1].p, nc, set[1].tmp, nr, ki);  // A p_right
         while (iterations < max_cg) {
-            cg.step(set[0]);
-            cg.step(set[1]);
+            if (paired) {
+                const BlockIO first[2] = {{set[0].p, nr, set[0].tmp, nc}, {set[1].tmp, nr, set[1].ap, nc}};
+                apply_handle_blocks(*m, true, first, 2, ki);
+                cg.project(set[1], set[1].p);
+                cg.update(set[1]);
+                const BlockIO second[2] = {{set[0].tmp, nc, set[0].ap, nr}, {set[1].p, nc, set[1].tmp, nr}};
+                apply_handle_blocks(*m, false, second, 2, ki);
+                cg.project(set[0], set[0].p);
+                cg.update(set[0]);
+            } else {
+                cg.step(set[0]);
+                cg.step(set[1]);
+            }
             ++iterations;
             queue_check(slot ^ 1);
             if (converged(slot)) break;

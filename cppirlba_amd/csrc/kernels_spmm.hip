@@ -126,18 +126,20 @@ void spmm_dispatch(const TiledLines& t, const void* X, int kp, int lpe, double* 
     }
 }
 
-// X_rm[r * kp + v] = xscale[r] * X[r + v * ld] (0 for k <= v < kp).
+// X_rm[r * kp + v] = xscale[r] * X[r + v * ld] for v < k, 0 for k <= v < kw
+// (`out` points at the first of the kw packed columns this call writes, so
+// several inputs can share one packed block).
 template <typename FT>
-__global__ void k_spmm_stage(const double* __restrict__ X, i64 ld, i64 L, int k, int kp,
+__global__ void k_spmm_stage(const double* __restrict__ X, i64 ld, i64 L, int k, int kw, int kp,
                              const double* __restrict__ xscale, FT* __restrict__ out) {
-    const i64 n = L * kp;
+    const i64 n = L * kw;
     for (i64 e = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
          e += static_cast<i64>(gridDim.x) * blockDim.x) {
-        const i64 r = e / kp;
-        const int v = static_cast<int>(e - r * kp);
+        const i64 r = e / kw;
+        const int v = static_cast<int>(e - r * kw);
         double x = v < k ? X[r + static_cast<i64>(v) * ld] : 0.0;
         if (xscale && v < k) x *= xscale[r];
-        out[e] = static_cast<FT>(x);
+        out[r * kp + v] = static_cast<FT>(x);
     }
 }
 
@@ -316,16 +318,17 @@ int spmm_padded_k(int k, bool f32, int* lanes_per_entry) {
     return lpe * vn;
 }
 
-void launch_spmm_stage(const double* X, i64 ld, i64 L, int k, int kp, const double* xscale, bool f32, void* out,
-                       hipStream_t s) {
-    if (L <= 0) return;
-    const i64 grid = std::max<i64>(1, std::min<i64>((L * kp + 255) / 256, 16384));
+void launch_spmm_stage(const double* X, i64 ld, i64 L, int k, int c0, int kw, int kp, const double* xscale, bool f32,
+                       void* out, hipStream_t s) {
+    if (L <= 0 || kw <= 0) return;
+    if (k > kw || c0 < 0 || c0 + kw > kp) throw std::runtime_error("packed block columns out of range");
+    const i64 grid = std::max<i64>(1, std::min<i64>((L * kw + 255) / 256, 16384));
     if (f32)
-        hipLaunchKernelGGL(k_spmm_stage<float>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, X, ld, L, k, kp,
-                           xscale, static_cast<float*>(out));
+        hipLaunchKernelGGL(k_spmm_stage<float>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, X, ld, L, k, kw,
+                           kp, xscale, static_cast<float*>(out) + c0);
     else
-        hipLaunchKernelGGL(k_spmm_stage<double>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, X, ld, L, k, kp,
-                           xscale, static_cast<double*>(out));
+        hipLaunchKernelGGL(k_spmm_stage<double>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, X, ld, L, k, kw,
+                           kp, xscale, static_cast<double*>(out) + c0);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
