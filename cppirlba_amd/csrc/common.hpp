@@ -306,19 +306,21 @@ void launch_coldots(const double* A, i64 lda, const double* B, i64 ldb, int k, i
 void launch_cols_axpby(double* out, i64 ldo, const double* a_coef, double a_scale, const double* x, i64 ldx,
                        const double* b_coef, double b_scale, int k, i64 L, hipStream_t s);
 void launch_cg_coefs(const double* a, double* b, double* out, int k, int mode, hipStream_t s);
-// Fused lock-step CG steps over k columns of length L (column stride L):
-// ap = uc - ap + sig2 in; {x += alpha p, r -= alpha ap, rr_new = |r|^2} with
-// alpha = rr / pap; p = beta p + r with beta = rr_new / rr.
+// ap = uc - ap + sig2 in over k columns of length L (column stride L).
 void launch_cg_op_finish(double* ap, const double* uc, const double* in, const double* sig2, i64 L, int k,
                          hipStream_t s);
-void launch_cg_update_xr(double* x, double* r, const double* p, const double* ap, const double* rr, const double* pap,
-                         i64 L, int k, double* partials, double* rr_new, hipStream_t s);
-void launch_cg_update_p(double* p, const double* r, const double* rr_new, const double* rr, i64 L, int k,
-                        hipStream_t s);
 void launch_backward_coefs(const double* J, const double* K, int ldy, const double* sv, const double* w, int k,
                            int complement, double* CX, double* CY, double* d, double* sig2, int* flag,
                            hipStream_t s);
 void launch_axpby(double* out, const double* a, double alpha, const double* b, double beta, i64 n, hipStream_t s);
+// The lock-step CG's update after the projection terms (kernels_backward.hip):
+// ap = uc - ap + sig2 p, alpha = rr / p'ap, x += alpha p, r -= alpha ap,
+// rr_new = r'r, p = (rr_new / rr) p + r, for k columns of length L.
+// `partials` holds 2 * cgf_chunks(L) * k doubles.
+int cgf_chunks(i64 L);
+void launch_cg_fused(double* ap, const double* uc, double* p, double* x, double* r, const double* sig2,
+                     const double* rr, double* rr_new, i64 L, int k, double* partials, hipStream_t s);
+
 void launch_fill(double* out, double v, i64 n, hipStream_t s);
 
 // Multi-vector products on the tiled layout (kernels_spmm.hip).  X packed

@@ -1491,16 +1491,17 @@ struct LockstepCg {
         launch_coldots(c.r, c.L, c.r, c.L, k, c.L, c.partials, c.rr, s);
     }
     void step(CgSet& c) {
-        op(c, c.p);
-        update(c);
+        apply_handle_multi(*m, c.left, c.p, c.L, c.tmp, c.Lo, k);
+        apply_handle_multi(*m, !c.left, c.tmp, c.Lo, c.ap, c.L, k);
+        project_update(c);
     }
-    // The CG update once ap = op(p).
-    void update(CgSet& c) {
-        launch_coldots(c.p, c.L, c.ap, c.L, k, c.L, c.partials, c.pap, s);
-        // alpha = rr / pap (0 where pap <= 0); x += alpha p; r -= alpha ap; rr_new = |r|^2
-        launch_cg_update_xr(c.x, c.r, c.p, c.ap, c.rr, c.pap, c.L, k, c.partials, c.rr_new, s);
-        // beta = rr_new / rr; p = beta p + r; rr_new becomes rr (the two buffers swap)
-        launch_cg_update_p(c.p, c.r, c.rr_new, c.rr, c.L, k, s);
+    // ap holds B p: the projection terms, then the CG update (fused kernels:
+    // alpha = rr / p'ap, x += alpha p, r -= alpha ap, beta = rr_new / rr,
+    // p = beta p + r; rr_new becomes rr, the two buffers swap).
+    void project_update(CgSet& c) {
+        launch_gram(c.basis, c.L, c.ap, c.L, k, c.L, c.partials, c.proj, ldy, s);
+        launch_rotate(c.basis, c.L, k, c.proj, ldy, k, c.uc, c.L, c.L, s);
+        launch_cg_fused(c.ap, c.uc, c.p, c.x, c.r, c.sig2, c.rr, c.rr_new, c.L, k, c.partials, s);
         std::swap(c.rr, c.rr_new);
     }
     void finish(CgSet& c) {
@@ -1681,12 +1682,10 @@ int run_svd_backward(const irlba_hip_matrix* m, i64 k, const double* U, const do
             if (paired) {
                 const BlockIO first[2] = {{set[0].p, nr, set[0].tmp, nc}, {set[1].tmp, nr, set[1].ap, nc}};
                 apply_handle_blocks(*m, true, first, 2, ki);
-                cg.project(set[1], set[1].p);
-                cg.update(set[1]);
+                cg.project_update(set[1]);
                 const BlockIO second[2] = {{set[0].tmp, nc, set[0].ap, nr}, {set[1].p, nc, set[1].tmp, nr}};
                 apply_handle_blocks(*m, false, second, 2, ki);
-                cg.project(set[0], set[0].p);
-                cg.update(set[0]);
+                cg.project_update(set[0]);
             } else {
                 cg.step(set[0]);
                 cg.step(set[1]);

@@ -385,13 +385,9 @@ void launch_cg_coefs(const double* a, double* b, double* out, int k, int mode, h
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
-// Fused steps of the lock-step CG (one kernel where four or two were), with
-// the arithmetic of the unfused forms, so the results are the same bit for bit.
-//   k_cg_op_finish: ap[:, i] = (basis (basis' ap))[:, i] - ap[:, i] + sig2[i] in[:, i]
-//   k_cg_update_xr: alpha_i = rr_i / pap_i (0 unless pap_i > 0); x += alpha p;
-//                   r -= alpha ap; per-block partial sums of r_i^2 in the layout
-//                   of k_coldots (grid gram_blocks(L) x k), reduced by k_coldots_reduce
-//   k_cg_update_p:  beta_i = rr_new_i / rr_i (0 unless rr_i > 0); p = p beta + r
+// The CG operator's last step (the starting residual's; the iterations fuse
+// it with the update, below):
+//   ap[:, i] = (basis (basis' ap))[:, i] - ap[:, i] + sig2[i] in[:, i]
 __global__ void k_cg_op_finish(double* __restrict__ ap, const double* __restrict__ uc, const double* __restrict__ in,
                                const double* __restrict__ sig2, i64 L, int k) {
     const i64 n = L * k;
@@ -400,39 +396,6 @@ __global__ void k_cg_op_finish(double* __restrict__ ap, const double* __restrict
         const int i = static_cast<int>(e / L);
         const double t = ap[e] * -1.0 + uc[e] * 1.0;
         ap[e] = t * 1.0 + in[e] * (sig2[i] * 1.0);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_cg_update_xr(double* __restrict__ x, double* __restrict__ r,
-                                                      const double* __restrict__ p, const double* __restrict__ ap,
-                                                      const double* __restrict__ rr, const double* __restrict__ pap,
-                                                      i64 L, i64 chunk, double* __restrict__ partials) {
-    __shared__ double red[8];
-    const int i = blockIdx.y;
-    const double pv = pap[i];
-    const double alpha = pv > 0 ? rr[i] / pv : 0.0;
-    const i64 r0 = static_cast<i64>(blockIdx.x) * chunk, r1 = std::min<i64>(L, r0 + chunk);
-    const i64 off = static_cast<i64>(i) * L;
-    double s = 0, unused = 0;
-    for (i64 l = r0 + threadIdx.x; l < r1; l += blockDim.x) {
-        x[off + l] = x[off + l] * 1.0 + p[off + l] * (alpha * 1.0);
-        const double rv = r[off + l] * 1.0 + ap[off + l] * (alpha * -1.0);
-        r[off + l] = rv;
-        s += rv * rv;
-    }
-    block_sum2(s, unused, red);
-    if (threadIdx.x == 0) partials[static_cast<i64>(i) * gridDim.x + blockIdx.x] = s;
-}
-
-__global__ void k_cg_update_p(double* __restrict__ p, const double* __restrict__ r, const double* __restrict__ rr_new,
-                              const double* __restrict__ rr, i64 L, int k) {
-    const i64 n = L * k;
-    for (i64 e = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
-         e += static_cast<i64>(gridDim.x) * blockDim.x) {
-        const int i = static_cast<int>(e / L);
-        const double rv = rr[i];
-        const double beta = rv > 0 ? rr_new[i] / rv : 0.0;
-        p[e] = p[e] * (beta * 1.0) + r[e] * 1.0;
     }
 }
 
@@ -446,22 +409,100 @@ void launch_cg_op_finish(double* ap, const double* uc, const double* in, const d
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
-void launch_cg_update_xr(double* x, double* r, const double* p, const double* ap, const double* rr, const double* pap,
-                         i64 L, int k, double* partials, double* rr_new, hipStream_t s) {
-    if (k <= 0) return;
-    const int nb = gram_blocks(L);
-    const i64 chunk = (L + nb - 1) / nb;
-    hipLaunchKernelGGL(k_cg_update_xr, dim3(nb, k), dim3(256), 0, s, x, r, p, ap, rr, pap, L, chunk, partials);
-    IRLBA_HIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_coldots_reduce, dim3(k), dim3(256), 0, s, partials, nb, k, rr_new);
-    IRLBA_HIP_LAUNCH_CHECK();
+// ------------------------------------------------------------------ fused lock-step CG
+// The CG's vector work after the Gram product proj = basis' (B p) and the
+// rotation uc = basis proj in three kernels instead of six; each kernel's
+// blocks sum the previous kernel's per-block partials themselves, in the same
+// fixed order in every block, so no separate reduction kernels run.  All three
+// use one grid: blocks (c, i) over row chunks c of the columns i.
+//   k_cgf_finish_dot: ap = uc - ap + sig2 p (= sig2 p - P B p); partials of p'ap
+//   k_cgf_xr:         alpha = rr / p'ap (0 unless p'ap > 0); x += alpha p;
+//                     r -= alpha ap; partials of r'r
+//   k_cgf_p:          beta = rr_new / rr (0 unless rr > 0); p = beta p + r;
+//                     the first chunk's blocks store rr_new
+// The element arithmetic is that of k_cg_op_finish and of the separate
+// update kernels these replaced; the sums run over other partitions.
+constexpr int kCgfMaxChunks = 64;  // row chunks per column
+int cgf_chunks(i64 L) { return static_cast<int>(std::max<i64>(1, std::min<i64>(kCgfMaxChunks, (L + 1023) / 1024))); }
+
+// The sum over b < nb of part[b] for one column: thread t adds b = t, t + 256,
+// ..., then the block sum (wave sums, then the waves in order).
+__device__ __forceinline__ double cgf_column_total(const double* __restrict__ part, int nb, double* red) {
+    double v = 0, unused = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v += part[b];
+    block_sum2(v, unused, red);
+    return v;
 }
 
-void launch_cg_update_p(double* p, const double* r, const double* rr_new, const double* rr, i64 L, int k,
-                        hipStream_t s) {
+__global__ __launch_bounds__(256) void k_cgf_finish_dot(double* __restrict__ ap, const double* __restrict__ uc,
+                                                        const double* __restrict__ p, const double* __restrict__ sig2,
+                                                        i64 L, i64 chunk, double* __restrict__ part) {
+    __shared__ double red[2 * 16];
+    const int i = blockIdx.y;
+    const double sg = sig2[i];
+    const i64 r0 = static_cast<i64>(blockIdx.x) * chunk, r1 = std::min<i64>(L, r0 + chunk);
+    const i64 off = static_cast<i64>(i) * L;
+    double s = 0, unused = 0;
+    for (i64 l = r0 + threadIdx.x; l < r1; l += blockDim.x) {
+        const double t = ap[off + l] * -1.0 + uc[off + l] * 1.0;
+        const double pe = p[off + l];
+        const double a = t * 1.0 + pe * (sg * 1.0);
+        ap[off + l] = a;
+        s += pe * a;
+    }
+    block_sum2(s, unused, red);
+    if (threadIdx.x == 0) part[static_cast<i64>(i) * gridDim.x + blockIdx.x] = s;
+}
+
+// Block (c, i): row chunk c of column i.
+__global__ __launch_bounds__(256) void k_cgf_xr(double* __restrict__ x, double* __restrict__ r,
+                                                const double* __restrict__ p, const double* __restrict__ ap,
+                                                const double* __restrict__ rr, const double* __restrict__ part_in,
+                                                i64 L, i64 chunk, double* __restrict__ part_out) {
+    __shared__ double red[2 * 16];
+    const int i = blockIdx.y;
+    const double pv = cgf_column_total(part_in + static_cast<i64>(i) * gridDim.x, gridDim.x, red);
+    const double alpha = pv > 0 ? rr[i] / pv : 0.0;
+    const i64 r0 = static_cast<i64>(blockIdx.x) * chunk, r1 = std::min<i64>(L, r0 + chunk);
+    const i64 off = static_cast<i64>(i) * L;
+    double s = 0, unused = 0;
+    for (i64 l = r0 + threadIdx.x; l < r1; l += blockDim.x) {
+        x[off + l] = x[off + l] * 1.0 + p[off + l] * (alpha * 1.0);
+        const double rv = r[off + l] * 1.0 + ap[off + l] * (alpha * -1.0);
+        r[off + l] = rv;
+        s += rv * rv;
+    }
+    block_sum2(s, unused, red);
+    if (threadIdx.x == 0) part_out[static_cast<i64>(i) * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_cgf_p(double* __restrict__ p, const double* __restrict__ r,
+                                               const double* __restrict__ part_in, const double* __restrict__ rr,
+                                               double* __restrict__ rr_new, i64 L, i64 chunk) {
+    __shared__ double red[2 * 16];
+    const int i = blockIdx.y;
+    const double rn = cgf_column_total(part_in + static_cast<i64>(i) * gridDim.x, gridDim.x, red);
+    if (blockIdx.x == 0 && threadIdx.x == 0) rr_new[i] = rn;
+    const double rv = rr[i];
+    const double beta = rv > 0 ? rn / rv : 0.0;
+    const i64 r0 = static_cast<i64>(blockIdx.x) * chunk, r1 = std::min<i64>(L, r0 + chunk);
+    const i64 off = static_cast<i64>(i) * L;
+    for (i64 l = r0 + threadIdx.x; l < r1; l += blockDim.x) p[off + l] = p[off + l] * (beta * 1.0) + r[off + l] * 1.0;
+}
+
+void launch_cg_fused(double* ap, const double* uc, double* p, double* x, double* r, const double* sig2,
+                     const double* rr, double* rr_new, i64 L, int k, double* partials, hipStream_t s) {
     if (L <= 0 || k <= 0) return;
-    hipLaunchKernelGGL(k_cg_update_p, dim3(static_cast<unsigned>(cols_grid(L, k))), dim3(256), 0, s, p, r, rr_new, rr,
-                       L, k);
+    const int nc = cgf_chunks(L);
+    const i64 chunk = (L + nc - 1) / nc;
+    double* part_a = partials;                             // p'ap per (column, chunk)
+    double* part_b = partials + static_cast<i64>(nc) * k;  // r'r per (column, chunk)
+    const dim3 grid(nc, k);
+    hipLaunchKernelGGL(k_cgf_finish_dot, grid, dim3(256), 0, s, ap, uc, p, sig2, L, chunk, part_a);
+    IRLBA_HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_cgf_xr, grid, dim3(256), 0, s, x, r, p, ap, rr, part_a, L, chunk, part_b);
+    IRLBA_HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_cgf_p, grid, dim3(256), 0, s, p, r, part_b, rr, rr_new, L, chunk);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 

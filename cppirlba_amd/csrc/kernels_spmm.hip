@@ -128,7 +128,8 @@ void spmm_dispatch(const TiledLines& t, const void* X, int kp, int lpe, double* 
 
 // X_rm[r * kp + v] = xscale[r] * X[r + v * ld] for v < k, 0 for k <= v < kw
 // (`out` points at the first of the kw packed columns this call writes, so
-// several inputs can share one packed block).
+// several inputs can share one packed block).  Consecutive threads write
+// consecutive packed values.
 template <typename FT>
 __global__ void k_spmm_stage(const double* __restrict__ X, i64 ld, i64 L, int k, int kw, int kp,
                              const double* __restrict__ xscale, FT* __restrict__ out) {
@@ -143,20 +144,40 @@ __global__ void k_spmm_stage(const double* __restrict__ X, i64 ld, i64 L, int k,
     }
 }
 
+// The finishing transpose goes through LDS in tiles of kTransRows rows: the
+// row-major packed values are read a packed row at a time and the column-major
+// output written kTransRows consecutive rows at a time, both coalesced (the LDS
+// row stride of k + 1 spreads a column over the banks).
+constexpr int kTransRows = 64;
+
 // Y[m + v * ldy] = oscale[m] * Y_rm[m * kp + v] - sum_j tvec_j[m] * gam[j * k + v].
-__global__ void k_spmm_finish(const double* __restrict__ Yrm, i64 n, int k, int kp, const double* __restrict__ oscale,
-                              int n_terms, const double* __restrict__ t0, const double* __restrict__ t1,
-                              const double* __restrict__ gam, double* __restrict__ Y, i64 ldy) {
-    const i64 total = n * k;
-    for (i64 e = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
-         e += static_cast<i64>(gridDim.x) * blockDim.x) {
-        const int v = static_cast<int>(e / n);
-        const i64 m = e - static_cast<i64>(v) * n;
-        double o = Yrm[m * kp + v];
-        if (oscale) o *= oscale[m];
-        if (n_terms > 0) o -= (t0 ? t0[m] : 1.0) * gam[v];
-        if (n_terms > 1) o -= (t1 ? t1[m] : 1.0) * gam[k + v];
-        Y[m + static_cast<i64>(v) * ldy] = o;
+__global__ __launch_bounds__(256) void k_spmm_finish(const double* __restrict__ Yrm, i64 n, int k, int kp,
+                                                     const double* __restrict__ oscale, int n_terms,
+                                                     const double* __restrict__ t0, const double* __restrict__ t1,
+                                                     const double* __restrict__ gam, double* __restrict__ Y,
+                                                     i64 ldy) {
+    extern __shared__ double tile[];
+    const int ts = k + 1;
+    for (i64 m0 = static_cast<i64>(blockIdx.x) * kTransRows; m0 < n;
+         m0 += static_cast<i64>(gridDim.x) * kTransRows) {
+        for (int e = threadIdx.x; e < kTransRows * k; e += blockDim.x) {
+            const int rr = e / k, v = e - rr * k;
+            const i64 m = m0 + rr;
+            if (m < n) tile[rr * ts + v] = Yrm[m * kp + v];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < kTransRows * k; e += blockDim.x) {
+            const int rr = e % kTransRows, v = e / kTransRows;
+            const i64 m = m0 + rr;
+            if (m < n) {
+                double o = tile[rr * ts + v];
+                if (oscale) o *= oscale[m];
+                if (n_terms > 0) o -= (t0 ? t0[m] : 1.0) * gam[v];
+                if (n_terms > 1) o -= (t1 ? t1[m] : 1.0) * gam[k + v];
+                Y[m + static_cast<i64>(v) * ldy] = o;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -345,8 +366,10 @@ void launch_spmm_lines(const TiledLines& t, const void* X, int k, bool x_f32, do
 void launch_spmm_finish(const double* Yrm, i64 n, int k, int kp, const double* oscale, int n_terms,
                         const double* t0, const double* t1, const double* gam, double* Y, i64 ldy, hipStream_t s) {
     if (n <= 0 || k <= 0) return;
-    const i64 grid = std::max<i64>(1, std::min<i64>((n * k + 255) / 256, 16384));
-    hipLaunchKernelGGL(k_spmm_finish, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, Yrm, n, k, kp, oscale,
+    if (k > kp) throw std::runtime_error("packed block columns out of range");
+    const i64 grid = std::max<i64>(1, std::min<i64>((n + kTransRows - 1) / kTransRows, 8192));
+    const size_t lds = sizeof(double) * kTransRows * (k + 1);
+    hipLaunchKernelGGL(k_spmm_finish, dim3(static_cast<unsigned>(grid)), dim3(256), lds, s, Yrm, n, k, kp, oscale,
                        n_terms, t0, t1, gam, Y, ldy);
     IRLBA_HIP_LAUNCH_CHECK();
 }
