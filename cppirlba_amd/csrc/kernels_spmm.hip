@@ -182,72 +182,95 @@ __global__ __launch_bounds__(256) void k_spmm_finish(const double* __restrict__ 
 }
 
 // ------------------------------------------------------------------ dense blocks
-// Dense storage loads one element at a time here (the block products serve the
-// small dense cases of the backward's CG, where launches, not bytes, dominate).
-template <typename VT>
-__device__ __forceinline__ double dense_at(const VT* A, i64 i) {
-    return to_double_any(A[i]);
-}
+// Y = A X for a dense A (lines of length Ln, contiguous) and kp packed vectors,
+// as a tiled product with FP64 accumulation: output row p, reduction index q,
+//   dot orientation (the product along A's lines):  Y[p] = sum_q A[p * Ln + q] X[q]
+//   axpy orientation (across A's lines):            Y[p] = sum_q A[q * Ln + p] X[q]
+// A block owns TP output rows and one of NS slices of the reduction; per chunk
+// of TQ reduction indices it stages the A tile and the X rows in LDS (as
+// FP64), and each thread accumulates OPT consecutive outputs of one row.  With
+// NS > 1 the slices' partial sums land in `scratch` and k_dense_block_fold adds
+// them in slice order.
+constexpr int kDenseTQ = 32;
 
-// Y[m * kp + v] = sum_j A[m, j] X[j * kp + v] over lines m of length Ln (A's
-// lines contiguous): one block per pair of lines, threads over j, a block-wide
-// reduction of the 2 x kp sums.
-template <typename VT, typename FT, int KP>
-__global__ __launch_bounds__(256) void k_dense_block_dot(const VT* __restrict__ A, i64 n_lines, i64 Ln,
-                                                         const FT* __restrict__ X, double* __restrict__ Y) {
-    __shared__ double red[4][2 * KP];
-    const i64 m0 = static_cast<i64>(blockIdx.x) * 2;
-    const bool two = m0 + 1 < n_lines;
-    double acc[2][KP];
+template <int KP>
+struct DenseTile {
+    static constexpr int OPT = KP >= 4 ? KP / 4 : 1;  // outputs per thread
+    static constexpr int TPR = KP / OPT;              // threads per output row
+    static constexpr int TP = 256 / TPR;              // output rows per block
+};
+
+template <typename VT, typename FT, int KP, bool DOT>
+__global__ __launch_bounds__(256) void k_dense_block(const VT* __restrict__ A, i64 Ln, i64 P, i64 Q, i64 slice,
+                                                     const FT* __restrict__ X, double* __restrict__ Y) {
+    typedef DenseTile<KP> T;
+    constexpr int AN = T::TP * kDenseTQ / 256;         // A tile elements per thread
+    constexpr int XN = (kDenseTQ * KP + 255) / 256;    // X tile elements per thread
+    __shared__ double As[T::TP * (kDenseTQ + 1)];
+    __shared__ double Xs[kDenseTQ * (KP + 1)];
+    const int tr = threadIdx.x / T::TPR, tc = threadIdx.x % T::TPR;
+    const i64 p0 = static_cast<i64>(blockIdx.x) * T::TP;
+    const i64 q_begin = static_cast<i64>(blockIdx.y) * slice, q_end = min(Q, q_begin + slice);
+    // The next chunk's tile is loaded into registers while the current one is
+    // multiplied out of LDS; each load of a chunk is issued before the first
+    // use (absent elements read element 0 and become 0), so a chunk waits for
+    // at most one memory latency.  DOT: consecutive threads along a line (q);
+    // else along p: both contiguous in A.
+    double av[AN], xv[XN];
+    auto fetch = [&](i64 q0) {
+        const int nq = static_cast<int>(min(static_cast<i64>(kDenseTQ), q_end - q0));
 #pragma unroll
-    for (int v = 0; v < KP; ++v) acc[0][v] = acc[1][v] = 0;
-    for (i64 j = threadIdx.x; j < Ln; j += 256) {
-        const double a0 = dense_at(A, m0 * Ln + j);
-        const double a1 = two ? dense_at(A, (m0 + 1) * Ln + j) : 0.0;
-        const FT* xr = X + j * KP;
+        for (int u = 0; u < AN; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            const int pr = DOT ? e / kDenseTQ : e % T::TP, qq = DOT ? e % kDenseTQ : e / T::TP;
+            const i64 p = p0 + pr, q = q0 + qq;
+            const bool in = p < P && qq < nq;
+            const double a = to_double_any(A[in ? (DOT ? p * Ln + q : q * Ln + p) : 0]);
+            av[u] = in ? a : 0.0;
+        }
 #pragma unroll
-        for (int v = 0; v < KP; ++v) {
-            const double x = static_cast<double>(xr[v]);
-            acc[0][v] += a0 * x;
-            acc[1][v] += a1 * x;
+        for (int u = 0; u < XN; ++u) {
+            const int e = threadIdx.x + 256 * u, qq = e / KP;
+            const bool in = e < kDenseTQ * KP && qq < nq;
+            const double x = static_cast<double>(X[in ? (q0 + qq) * KP + (e - qq * KP) : 0]);
+            xv[u] = in ? x : 0.0;
+        }
+    };
+    double acc[T::OPT];
+#pragma unroll
+    for (int o = 0; o < T::OPT; ++o) acc[o] = 0;
+    if (q_begin < q_end) fetch(q_begin);
+    for (i64 q0 = q_begin; q0 < q_end; q0 += kDenseTQ) {
+        __syncthreads();  // the previous chunk's tiles are consumed
+#pragma unroll
+        for (int u = 0; u < AN; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            const int pr = DOT ? e / kDenseTQ : e % T::TP, qq = DOT ? e % kDenseTQ : e / T::TP;
+            As[pr * (kDenseTQ + 1) + qq] = av[u];
+        }
+#pragma unroll
+        for (int u = 0; u < XN; ++u) {
+            const int e = threadIdx.x + 256 * u, qq = e / KP;
+            if (e < kDenseTQ * KP) Xs[qq * (KP + 1) + (e - qq * KP)] = xv[u];
+        }
+        __syncthreads();
+        if (q0 + kDenseTQ < q_end) fetch(q0 + kDenseTQ);
+        const double* ar = As + tr * (kDenseTQ + 1);
+        const double* xc = Xs + tc * T::OPT;
+        // Absent reduction indices of the last chunk hold zeros in both tiles.
+#pragma unroll 8
+        for (int qq = 0; qq < kDenseTQ; ++qq) {
+            const double a = ar[qq];
+#pragma unroll
+            for (int o = 0; o < T::OPT; ++o) acc[o] += a * xc[qq * (KP + 1) + o];
         }
     }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const i64 p = p0 + tr;
+    if (p < P && tc * T::OPT < KP) {
+        double* out = Y + static_cast<i64>(blockIdx.y) * P * KP + p * KP + tc * T::OPT;
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int v = 0; v < KP; ++v) {
-            const double t = wave_sum(acc[r][v]);
-            if (lane == 0) red[w][r * KP + v] = t;
-        }
-    __syncthreads();
-    if (threadIdx.x < 2 * KP) {
-        const int r = threadIdx.x / KP, v = threadIdx.x - r * KP;
-        if (r == 0 || two)
-            Y[(m0 + r) * KP + v] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+        for (int o = 0; o < T::OPT; ++o) out[o] = acc[o];
     }
-}
-
-// P[c][i * kp + v] = sum over lines l of chunk c of A[l, i] U[l * kp + v]
-// (thread per i); the chunks are summed by k_dense_block_fold.
-template <typename VT, typename FT, int KP>
-__global__ __launch_bounds__(256) void k_dense_block_axpy(const VT* __restrict__ A, i64 n_lines, i64 Ln, i64 chunk,
-                                                          const FT* __restrict__ U, double* __restrict__ P) {
-    const i64 i = static_cast<i64>(blockIdx.x) * 256 + threadIdx.x;
-    const i64 l0 = static_cast<i64>(blockIdx.y) * chunk, l1 = min(n_lines, l0 + chunk);
-    if (i >= Ln) return;
-    double acc[KP];
-#pragma unroll
-    for (int v = 0; v < KP; ++v) acc[v] = 0;
-    for (i64 l = l0; l < l1; ++l) {
-        const double a = dense_at(A, l * Ln + i);
-        const FT* ur = U + l * KP;  // the same row for every thread: a broadcast load
-#pragma unroll
-        for (int v = 0; v < KP; ++v) acc[v] += a * static_cast<double>(ur[v]);
-    }
-    double* out = P + static_cast<i64>(blockIdx.y) * Ln * KP + i * KP;
-#pragma unroll
-    for (int v = 0; v < KP; ++v) out[v] = acc[v];
 }
 
 __global__ void k_dense_block_fold(const double* __restrict__ P, int chunks, i64 n, double* __restrict__ Y) {
@@ -259,38 +282,64 @@ __global__ void k_dense_block_fold(const double* __restrict__ P, int chunks, i64
     }
 }
 
+// Output rows, reduction length and reduction slices of a dense block product.
+struct DensePlan {
+    i64 P, Q, tiles, slices, slice;
+};
+
+template <int KP>
+DensePlan dense_plan(const DenseDesc& d, bool adjoint) {
+    const i64 n_lines = d.col_major ? d.nc : d.nr;
+    const i64 Ln = d.col_major ? d.nr : d.nc;
+    const bool dot = d.col_major ? adjoint : !adjoint;
+    DensePlan pl;
+    pl.P = dot ? n_lines : Ln;
+    pl.Q = dot ? Ln : n_lines;
+    pl.tiles = std::max<i64>(1, (pl.P + DenseTile<KP>::TP - 1) / DenseTile<KP>::TP);
+    if (pl.Q == 0) {  // no reduction: a zero product, written without the kernel
+        pl.slices = 1;
+        pl.slice = 0;
+        return pl;
+    }
+    // About 512 blocks in all, slices of at least two chunks.
+    const i64 max_slices = std::max<i64>(1, pl.Q / (2 * kDenseTQ));
+    pl.slices = std::max<i64>(1, std::min<i64>(max_slices, 512 / pl.tiles));
+    pl.slice = (pl.Q + pl.slices - 1) / pl.slices;
+    pl.slice = (pl.slice + kDenseTQ - 1) / kDenseTQ * kDenseTQ;
+    pl.slices = std::max<i64>(1, (pl.Q + pl.slice - 1) / pl.slice);
+    return pl;
+}
+
 template <typename VT, typename FT, int KP>
 void dense_block_typed(const DenseDesc& d, bool adjoint, const void* Xv, double* Y, double* scratch, i64 scratch_cap,
                        hipStream_t s) {
     const VT* A = static_cast<const VT*>(d.a);
     const FT* X = static_cast<const FT*>(Xv);
-    const i64 n_lines = d.col_major ? d.nc : d.nr;
     const i64 Ln = d.col_major ? d.nr : d.nc;
     const bool dot = d.col_major ? adjoint : !adjoint;
-    if (n_lines == 0 || Ln == 0) {  // no entries: a zero product
-        const i64 n_out = dot ? n_lines : Ln;
-        if (n_out > 0) IRLBA_HIP_CHECK(hipMemsetAsync(Y, 0, sizeof(double) * n_out * KP, s));
+    const DensePlan pl = dense_plan<KP>(d, adjoint);
+    if (pl.P == 0) return;
+    if (pl.Q == 0) {  // no entries: a zero product
+        launch_fill(Y, 0.0, pl.P * KP, s);
         return;
     }
-    if (dot) {
-        const i64 grid = (n_lines + 1) / 2;
-        hipLaunchKernelGGL((k_dense_block_dot<VT, FT, KP>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, A,
-                           n_lines, Ln, X, Y);
+    if (pl.slices > 1 && pl.slices * pl.P * KP > scratch_cap)
+        throw std::runtime_error("dense block product scratch too small");
+    double* out = pl.slices > 1 ? scratch : Y;
+    const dim3 grid(static_cast<unsigned>(pl.tiles), static_cast<unsigned>(pl.slices));
+    if (dot)
+        hipLaunchKernelGGL((k_dense_block<VT, FT, KP, true>), grid, dim3(256), 0, s, A, Ln, pl.P, pl.Q, pl.slice, X,
+                           out);
+    else
+        hipLaunchKernelGGL((k_dense_block<VT, FT, KP, false>), grid, dim3(256), 0, s, A, Ln, pl.P, pl.Q, pl.slice, X,
+                           out);
+    IRLBA_HIP_LAUNCH_CHECK();
+    if (pl.slices > 1) {
+        const i64 n = pl.P * KP;
+        hipLaunchKernelGGL(k_dense_block_fold, dim3(static_cast<unsigned>(std::min<i64>((n + 255) / 256, 4096))),
+                           dim3(256), 0, s, scratch, static_cast<int>(pl.slices), n, Y);
         IRLBA_HIP_LAUNCH_CHECK();
-        return;
     }
-    const i64 xb = (Ln + 255) / 256;
-    i64 chunks = std::max<i64>(1, std::min<i64>(std::max<i64>(1, 1024 / xb), (n_lines + 31) / 32));
-    chunks = std::min<i64>(chunks, std::max<i64>(1, scratch_cap / std::max<i64>(1, Ln * KP)));
-    const i64 chunk = (n_lines + chunks - 1) / chunks;
-    chunks = (n_lines + chunk - 1) / chunk;
-    hipLaunchKernelGGL((k_dense_block_axpy<VT, FT, KP>), dim3(static_cast<unsigned>(xb), static_cast<unsigned>(chunks)),
-                       dim3(256), 0, s, A, n_lines, Ln, chunk, X, scratch);
-    IRLBA_HIP_LAUNCH_CHECK();
-    const i64 n = Ln * KP;
-    hipLaunchKernelGGL(k_dense_block_fold, dim3(static_cast<unsigned>(std::min<i64>((n + 255) / 256, 4096))), dim3(256),
-                       0, s, scratch, static_cast<int>(chunks), n, Y);
-    IRLBA_HIP_LAUNCH_CHECK();
 }
 
 template <typename VT, typename FT>
@@ -309,11 +358,19 @@ void dense_block_kp(const DenseDesc& d, bool adjoint, int kp, const void* X, dou
 }  // namespace
 
 i64 dense_block_scratch(const DenseDesc& d, int kp) {
-    const i64 n_lines = d.col_major ? d.nc : d.nr;
-    const i64 Ln = d.col_major ? d.nr : d.nc;
-    const i64 xb = std::max<i64>(1, (Ln + 255) / 256);  // at least 1: a matrix may have no lines or no entries per line
-    const i64 chunks = std::max<i64>(1, std::min<i64>(std::max<i64>(1, 1024 / xb), (n_lines + 31) / 32));
-    return Ln * kp * chunks;  // the axpy orientation's partials (the dot one needs none)
+    // The larger of the two orientations' slice partials.
+    auto need = [&](bool adjoint) -> i64 {
+        DensePlan pl;
+        switch (kp) {
+            case 2: pl = dense_plan<2>(d, adjoint); break;
+            case 4: pl = dense_plan<4>(d, adjoint); break;
+            case 8: pl = dense_plan<8>(d, adjoint); break;
+            case 16: pl = dense_plan<16>(d, adjoint); break;
+            default: pl = dense_plan<32>(d, adjoint); break;
+        }
+        return pl.slices > 1 ? pl.slices * pl.P * kp : 0;
+    };
+    return std::max<i64>(1, std::max(need(false), need(true)));
 }
 
 void launch_dense_block(const DenseDesc& d, bool adjoint, int kp, bool x_f32, const void* X, double* Y,
