@@ -106,6 +106,19 @@ def cpu_baseline(nr, ncol_full, density, seed, nnz_full, n_fwd, n_adj, restarts,
     }
 
 
+def product_pair_bytes(h, value_bytes, nr, nc):
+    """Bytes one A.v and one A'.u of a sparse handle stream: every stored slot
+    (value + delta or 16-bit index), the segment pointers and first indices,
+    and the FP64 input and output vectors."""
+    total = 0
+    for adjoint in (False, True):
+        lay = h.layout(adjoint)
+        nseg = lay["n_tiles"] * (nc if adjoint else nr)
+        total += (lay["slots"] * value_bytes + lay["index_bytes"] + (nseg + 1) * 8 + (nseg * 2 if lay["delta"] else 0)
+                  + (nr + nc) * 8)
+    return total
+
+
 def spmv_mode(args):
     """Products only (for rocprofv3 counter runs): `reps` A.v and A'.u products of
     the mixed-precision matrix, then a device copy of a known size whose
@@ -557,6 +570,7 @@ def main():
         mixed.value_dictionary(False)
 
     if not args.skip_fp64:
+        mixed_pair_bytes = product_pair_bytes(mixed, 4, nr, nc)
         del mixed
         log("FP64 path on the same matrix (FP64 storage and accumulation)")
         fp64 = irlba.generate_sparse(nr, nc, args.density, args.seed, storage="f64", accumulate="f64")
@@ -584,6 +598,28 @@ def main():
         # restarts on this flat spectrum), so also the time per restart.
         out["mixed_vs_fp64_per_restart_speedup"] = round(
             (dt64 / max(1, res64.iterations)) / (float(np.mean(times)) / max(1, mixed_res.iterations)), 3)
+        # What bounds the per-restart ratio: the products (the only part whose
+        # bytes depend on the value precision) against their streamed bytes,
+        # and the FP64 work both runs share (Gram-Schmidt, rotations, norms).
+        ms_m = out["spmv"]["forward_ms"] + out["spmv"]["adjoint_ms"]
+        ms_d = out["fp64"]["forward_ms"] + out["fp64"]["adjoint_ms"]
+        pairs_m = min(out["spmv"]["forward_products"], out["spmv"]["adjoint_products"]) / max(1, mixed_res.iterations)
+        restart_m = float(np.mean(times)) * 1e3 / max(1, mixed_res.iterations)
+        restart_d = dt64 * 1e3 / max(1, res64.iterations)
+        b_m, b_d = mixed_pair_bytes, product_pair_bytes(fp64, 8, nr, nc)
+        out["mixed_vs_fp64_breakdown"] = {
+            "products_speedup": round(ms_d / ms_m, 3),
+            "product_bytes_ratio": round(b_d / b_m, 3),
+            "product_pair_bytes": {"mixed": int(b_m), "fp64": int(b_d)},
+            "product_pairs_per_restart": round(pairs_m, 2),
+            "shared_fp64_ms_per_restart": {"mixed": round(restart_m - pairs_m * ms_m, 3),
+                                           "fp64": round(restart_d - pairs_m * ms_d, 3)},
+            "note": "per restart both precisions run the same FP64 Gram-Schmidt, rotations and norms (FP64 by "
+                    "design); only the products change, and they stream FP32 value + 8-bit delta (5 bytes per "
+                    "slot) against FP64 value + delta (9 bytes), so the per-restart ratio stays below the products' "
+                    "own ratio; 2x per restart would need the products to beat their byte ratio by more than the "
+                    "shared work costs",
+        }
         out["sv_rel_err_vs_fp64"] = rel
         out["max_subspace_angle_vs_fp64"] = angle
         if dict_out is not None:
