@@ -186,18 +186,24 @@ __global__ __launch_bounds__(256) void k_spmm_finish(const double* __restrict__ 
 // as a tiled product with FP64 accumulation: output row p, reduction index q,
 //   dot orientation (the product along A's lines):  Y[p] = sum_q A[p * Ln + q] X[q]
 //   axpy orientation (across A's lines):            Y[p] = sum_q A[q * Ln + p] X[q]
-// A block owns TP output rows and one of NS slices of the reduction; per chunk
-// of TQ reduction indices it stages the A tile and the X rows in LDS (as
-// FP64), and each thread accumulates OPT consecutive outputs of one row.  With
+// A block owns TP = 256 output rows and one of NS slices of the reduction; per
+// chunk of TQ reduction indices it stages the A tile and the X rows in LDS (as
+// FP64; the next chunk is loaded into registers meanwhile), and each thread
+// accumulates RPT rows x OPT consecutive outputs (4 x 8 for 32 vectors: 4 + 8
+// LDS reads per 32 FMAs).  A thread's rows are r * (TP / RPT) + g, so the rows
+// a wavefront reads together are consecutive and fall in distinct banks.  With
 // NS > 1 the slices' partial sums land in `scratch` and k_dense_block_fold adds
 // them in slice order.
-constexpr int kDenseTQ = 32;
+constexpr int kDenseTQ = 16;
 
 template <int KP>
 struct DenseTile {
-    static constexpr int OPT = KP >= 4 ? KP / 4 : 1;  // outputs per thread
-    static constexpr int TPR = KP / OPT;              // threads per output row
-    static constexpr int TP = 256 / TPR;              // output rows per block
+    static constexpr int OPT = KP >= 4 ? KP / 4 : 1;  // outputs per thread and row
+    static constexpr int TPR = KP / OPT;              // threads per row group
+    static constexpr int RPT = TPR;                   // rows per thread
+    static constexpr int TP = 256;                    // output rows per block
+    static constexpr int G = TP / RPT;                // row groups
+    static_assert(G * TPR == 256, "a block is 256 threads");
 };
 
 template <typename VT, typename FT, int KP, bool DOT>
@@ -208,14 +214,13 @@ __global__ __launch_bounds__(256) void k_dense_block(const VT* __restrict__ A, i
     constexpr int XN = (kDenseTQ * KP + 255) / 256;    // X tile elements per thread
     __shared__ double As[T::TP * (kDenseTQ + 1)];
     __shared__ double Xs[kDenseTQ * (KP + 1)];
-    const int tr = threadIdx.x / T::TPR, tc = threadIdx.x % T::TPR;
+    const int g = threadIdx.x / T::TPR, tc = threadIdx.x % T::TPR;
     const i64 p0 = static_cast<i64>(blockIdx.x) * T::TP;
     const i64 q_begin = static_cast<i64>(blockIdx.y) * slice, q_end = min(Q, q_begin + slice);
-    // The next chunk's tile is loaded into registers while the current one is
-    // multiplied out of LDS; each load of a chunk is issued before the first
-    // use (absent elements read element 0 and become 0), so a chunk waits for
-    // at most one memory latency.  DOT: consecutive threads along a line (q);
-    // else along p: both contiguous in A.
+    // Each load of a chunk is issued before the first use (absent elements
+    // read element 0 and become 0), so a chunk waits for at most one memory
+    // latency.  DOT: consecutive threads along a line (q); else along p, both
+    // contiguous in A.
     double av[AN], xv[XN];
     auto fetch = [&](i64 q0) {
         const int nq = static_cast<int>(min(static_cast<i64>(kDenseTQ), q_end - q0));
@@ -236,9 +241,11 @@ __global__ __launch_bounds__(256) void k_dense_block(const VT* __restrict__ A, i
             xv[u] = in ? x : 0.0;
         }
     };
-    double acc[T::OPT];
+    double acc[T::RPT][T::OPT];
 #pragma unroll
-    for (int o = 0; o < T::OPT; ++o) acc[o] = 0;
+    for (int r = 0; r < T::RPT; ++r)
+#pragma unroll
+        for (int o = 0; o < T::OPT; ++o) acc[r][o] = 0;
     if (q_begin < q_end) fetch(q_begin);
     for (i64 q0 = q_begin; q0 < q_end; q0 += kDenseTQ) {
         __syncthreads();  // the previous chunk's tiles are consumed
@@ -255,21 +262,29 @@ __global__ __launch_bounds__(256) void k_dense_block(const VT* __restrict__ A, i
         }
         __syncthreads();
         if (q0 + kDenseTQ < q_end) fetch(q0 + kDenseTQ);
-        const double* ar = As + tr * (kDenseTQ + 1);
         const double* xc = Xs + tc * T::OPT;
         // Absent reduction indices of the last chunk hold zeros in both tiles.
-#pragma unroll 8
+#pragma unroll 4
         for (int qq = 0; qq < kDenseTQ; ++qq) {
-            const double a = ar[qq];
+            double a[T::RPT], x[T::OPT];
 #pragma unroll
-            for (int o = 0; o < T::OPT; ++o) acc[o] += a * xc[qq * (KP + 1) + o];
+            for (int r = 0; r < T::RPT; ++r) a[r] = As[(r * T::G + g) * (kDenseTQ + 1) + qq];
+#pragma unroll
+            for (int o = 0; o < T::OPT; ++o) x[o] = xc[qq * (KP + 1) + o];
+#pragma unroll
+            for (int r = 0; r < T::RPT; ++r)
+#pragma unroll
+                for (int o = 0; o < T::OPT; ++o) acc[r][o] += a[r] * x[o];
         }
     }
-    const i64 p = p0 + tr;
-    if (p < P && tc * T::OPT < KP) {
-        double* out = Y + static_cast<i64>(blockIdx.y) * P * KP + p * KP + tc * T::OPT;
 #pragma unroll
-        for (int o = 0; o < T::OPT; ++o) out[o] = acc[o];
+    for (int r = 0; r < T::RPT; ++r) {
+        const i64 p = p0 + r * T::G + g;
+        if (p < P) {
+            double* out = Y + static_cast<i64>(blockIdx.y) * P * KP + p * KP + tc * T::OPT;
+#pragma unroll
+            for (int o = 0; o < T::OPT; ++o) out[o] = acc[r][o];
+        }
     }
 }
 
@@ -301,9 +316,11 @@ DensePlan dense_plan(const DenseDesc& d, bool adjoint) {
         pl.slice = 0;
         return pl;
     }
-    // About 512 blocks in all, slices of at least two chunks.
+    // Up to 512 blocks, slices of at least two chunks, and at most 2^20 doubles
+    // of slice partials (the fold reads them all back).
     const i64 max_slices = std::max<i64>(1, pl.Q / (2 * kDenseTQ));
-    pl.slices = std::max<i64>(1, std::min<i64>(max_slices, 512 / pl.tiles));
+    const i64 fold_cap = std::max<i64>(1, (static_cast<i64>(1) << 20) / std::max<i64>(1, pl.P * KP));
+    pl.slices = std::max<i64>(1, std::min<i64>(std::min(max_slices, fold_cap), 512 / pl.tiles));
     pl.slice = (pl.Q + pl.slices - 1) / pl.slices;
     pl.slice = (pl.slice + kDenseTQ - 1) / kDenseTQ * kDenseTQ;
     pl.slices = std::max<i64>(1, (pl.Q + pl.slice - 1) / pl.slice);
