@@ -56,6 +56,43 @@ def test_dense_backward_vs_oracle(m, n, complement):
         assert res.cg_iterations == 0
 
 
+@pytest.mark.parametrize("storage,k", [("f64", 20), ("f32", 20), ("f64", 12)])
+def test_exact_backward_wider_k(storage, k):
+    # The lock-step CG shares each pass over the matrix between its two sets of
+    # systems when 2k vectors fit one pass (dense: 32 packed columns); k = 20
+    # runs the sets one after the other, k = 12 shares the passes.  A matrix
+    # with a gap under s_k keeps the systems well conditioned.
+    m, n = 300, 180
+    rng = np.random.default_rng(30 + k)
+    Q1, _ = np.linalg.qr(rng.standard_normal((m, k)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, k)))
+    A = (Q1 * (40.0 + np.arange(k)[::-1])) @ Q2.T + 0.3 * rng.standard_normal((m, n))
+    if storage == "f32":
+        A = A.astype(np.float32).astype(np.float64)  # the stored matrix
+    U, s, V = _triplets(A, k)
+    w, GU, GV = _grads(m, n, k, 31)
+    res = irlba.svd_backward(irlba.HipDenseMatrix(A, storage=storage), U, s, V, w, GU, GV, complement="exact")
+    Xo, Yo = OB.svd_backward(A, U, s, V, w, GU, GV, "exact")
+    assert res.cg_iterations > 0
+    # FP32 storage: the products see FP32-rounded inputs, so the CG converges to 1e-7 of the answer.
+    assert _close(res.dense(), Xo @ Yo.T, 1e-9 if storage == "f64" else 1e-6)
+
+
+def test_sparse_exact_backward_k40():
+    # 40 systems per set: the fused CG update kernels past 32 columns, and the
+    # shared passes over the matrix with 80 packed vectors.
+    nr, nc, k = 700, 400, 40
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.08, 41, f32=False)
+    A = _dense_of(vals, idx, ptr, nr, nc)
+    h = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, storage="f64")
+    U, s, V = _triplets(A, k)
+    w, GU, GV = _grads(nr, nc, k, 42)
+    res = irlba.svd_backward(h, U, s, V, w, GU, GV, complement="exact")
+    Xo, Yo = OB.svd_backward(A, U, s, V, w, GU, GV, "exact")
+    assert res.cg_iterations > 0
+    assert _close(res.dense(), Xo @ Yo.T, 1e-9)
+
+
 def test_dense_backward_vs_finite_differences():
     # 64 x 32 FP64, the whole gradient by central differences of a sign-fixed SVD.
     m, n, k = 64, 32, 5
