@@ -144,30 +144,30 @@ __global__ void k_spmm_stage(const double* __restrict__ X, i64 ld, i64 L, int k,
     }
 }
 
-// The finishing transpose goes through LDS in tiles of kTransRows rows: the
+// The finishing transpose goes through LDS in tiles of `rows` rows: the
 // row-major packed values are read a packed row at a time and the column-major
-// output written kTransRows consecutive rows at a time, both coalesced (the LDS
-// row stride of k + 1 spreads a column over the banks).
-constexpr int kTransRows = 64;
+// output written `rows` consecutive rows at a time, both coalesced (the LDS
+// row stride of k + 1 spreads a column over the banks).  Tiles hold 64 rows,
+// or 32 from k = 64 on, so a tile takes at most ~33 KB of LDS at k = 128.
+static int finish_rows(int k) { return k < 64 ? 64 : 32; }
 
 // Y[m + v * ldy] = oscale[m] * Y_rm[m * kp + v] - sum_j tvec_j[m] * gam[j * k + v].
 __global__ __launch_bounds__(256) void k_spmm_finish(const double* __restrict__ Yrm, i64 n, int k, int kp,
                                                      const double* __restrict__ oscale, int n_terms,
                                                      const double* __restrict__ t0, const double* __restrict__ t1,
                                                      const double* __restrict__ gam, double* __restrict__ Y,
-                                                     i64 ldy) {
+                                                     i64 ldy, int rows) {
     extern __shared__ double tile[];
     const int ts = k + 1;
-    for (i64 m0 = static_cast<i64>(blockIdx.x) * kTransRows; m0 < n;
-         m0 += static_cast<i64>(gridDim.x) * kTransRows) {
-        for (int e = threadIdx.x; e < kTransRows * k; e += blockDim.x) {
+    for (i64 m0 = static_cast<i64>(blockIdx.x) * rows; m0 < n; m0 += static_cast<i64>(gridDim.x) * rows) {
+        for (int e = threadIdx.x; e < rows * k; e += blockDim.x) {
             const int rr = e / k, v = e - rr * k;
             const i64 m = m0 + rr;
             if (m < n) tile[rr * ts + v] = Yrm[m * kp + v];
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < kTransRows * k; e += blockDim.x) {
-            const int rr = e % kTransRows, v = e / kTransRows;
+        for (int e = threadIdx.x; e < rows * k; e += blockDim.x) {
+            const int rr = e % rows, v = e / rows;
             const i64 m = m0 + rr;
             if (m < n) {
                 double o = tile[rr * ts + v];
@@ -441,10 +441,11 @@ void launch_spmm_finish(const double* Yrm, i64 n, int k, int kp, const double* o
                         const double* t0, const double* t1, const double* gam, double* Y, i64 ldy, hipStream_t s) {
     if (n <= 0 || k <= 0) return;
     if (k > kp) throw std::runtime_error("packed block columns out of range");
-    const i64 grid = std::max<i64>(1, std::min<i64>((n + kTransRows - 1) / kTransRows, 8192));
-    const size_t lds = sizeof(double) * kTransRows * (k + 1);
+    const int rows = finish_rows(k);
+    const i64 grid = std::max<i64>(1, std::min<i64>((n + rows - 1) / rows, 8192));
+    const size_t lds = sizeof(double) * rows * (k + 1);
     hipLaunchKernelGGL(k_spmm_finish, dim3(static_cast<unsigned>(grid)), dim3(256), lds, s, Yrm, n, k, kp, oscale,
-                       n_terms, t0, t1, gam, Y, ldy);
+                       n_terms, t0, t1, gam, Y, ldy, rows);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 

@@ -135,7 +135,7 @@ def _csc_dense(vals, idx, ptr, nr, nc):
 
 
 @pytest.mark.parametrize("storage,accumulate", [("f64", "f64"), ("f32", "f32"), ("f32", "f64")])
-@pytest.mark.parametrize("k", [1, 3, 16, 33])
+@pytest.mark.parametrize("k", [1, 3, 16, 33, 64, 128])
 @pytest.mark.parametrize("decorated", [False, True])
 def test_multiply_block_vs_numpy(storage, accumulate, k, decorated):
     # Block products on the tiled layout: 70000 columns span three FP32 tiles of
