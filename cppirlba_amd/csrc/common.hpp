@@ -324,15 +324,30 @@ void launch_cg_fused(double* ap, const double* uc, double* p, double* x, double*
 void launch_fill(double* out, double v, i64 n, hipStream_t s);
 
 // Multi-vector products on the tiled layout (kernels_spmm.hip).  X packed
-// row-major with stride spmm_padded_k(k, x_f32) (FP32 when x_f32), Y row-major
-// FP64 with the same stride.  A stage call fills packed columns [c0, c0 + kw)
-// from k <= kw input columns (zeros after them).
+// row-major with stride spmm_padded_k(n k, x_f32) (FP32 when x_f32), Y row-major
+// FP64 with the same stride; the n blocks of k column-major vectors sit side
+// by side, block b in packed columns [b k, (b + 1) k).  The stage fills every
+// packed column (zeros after the blocks), the finish unpacks every block, one
+// launch each.
+constexpr int kMaxPackBlocks = 4;
+struct StageBlocks {
+    const double* X[kMaxPackBlocks];
+    i64 ld[kMaxPackBlocks];
+    int n;
+};
+struct FinishBlocks {
+    double* Y[kMaxPackBlocks];
+    i64 ld[kMaxPackBlocks];
+    int n;
+};
 int spmm_padded_k(int k, bool x_f32, int* lanes_per_entry = nullptr);
-void launch_spmm_stage(const double* X, i64 ld, i64 L, int k, int c0, int kw, int kp, const double* xscale,
-                       bool x_f32, void* out, hipStream_t s);
+void launch_spmm_stage(const StageBlocks& in, i64 L, int k, int kp, const double* xscale, bool x_f32, void* out,
+                       hipStream_t s);
 void launch_spmm_lines(const TiledLines& t, const void* X, int k, bool x_f32, double* Y, hipStream_t s);
+// Y_b = oscale .* Y_rm[:, block b] - sum_j tvec_j gam[(2 b + j) k + :] for every block.
 void launch_spmm_finish(const double* Yrm, i64 n, int k, int kp, const double* oscale, int n_terms,
-                        const double* t0, const double* t1, const double* gam, double* Y, i64 ldy, hipStream_t s);
+                        const double* t0, const double* t1, const double* gam, const FinishBlocks& out,
+                        hipStream_t s);
 // Dense block products on packed inputs (kp <= 32); `scratch` (FP64) holds the
 // per-chunk partials of the axpy orientation, up to dense_block_scratch(d, kp).
 i64 dense_block_scratch(const DenseDesc& d, int kp);

@@ -397,6 +397,11 @@ bool blocks_fit_one_pass(const irlba_hip_matrix& m, bool adjoint, int k, int nio
 // (kernels_spmm.hip), the blocks packed side by side; blocks that do not fit
 // one pass run one at a time, and a single vector as a single product.
 void apply_handle_blocks(const irlba_hip_matrix& m, bool adjoint, const BlockIO* io, int nio, int k) {
+    if (nio > kMaxPackBlocks) {
+        apply_handle_blocks(m, adjoint, io, kMaxPackBlocks, k);
+        apply_handle_blocks(m, adjoint, io + kMaxPackBlocks, nio - kMaxPackBlocks, k);
+        return;
+    }
     if (nio > 1 && !blocks_fit_one_pass(m, adjoint, k, nio)) {
         for (int b = 0; b < nio; ++b) apply_handle_blocks(m, adjoint, io + b, 1, k);
         return;
@@ -445,24 +450,28 @@ void apply_handle_blocks(const irlba_hip_matrix& m, bool adjoint, const BlockIO*
     double* partials = static_cast<double*>(wp[3]);
     double* ones = static_cast<double*>(wp[4]);
     if (need_ones) launch_fill(ones, 1.0, L_in, s);
+    StageBlocks in{};
+    FinishBlocks out{};
+    in.n = out.n = nio;
     for (int b = 0; b < nio; ++b) {
         for (int j = 0; j < a.n_terms; ++j) {
             // gamma_j[v] = tdot_j . X[:, v] on the unscaled input (the single product's prologue).
             launch_cgs_dots(io[b].X, io[b].ldx, k, tdot[j] ? tdot[j] : ones, L_in, partials, s);
             launch_cgs_reduce(partials, L_in, k, gam + (2 * b + j) * k, s);
         }
-        // The last block also zeroes the padding columns.
-        launch_spmm_stage(io[b].X, io[b].ldx, L_in, k, b * k, b + 1 < nio ? k : kp - b * k, kp, xscale, x_f32, wp[0],
-                          s);
+        in.X[b] = io[b].X;
+        in.ld[b] = io[b].ldx;
+        out.Y[b] = io[b].Y;
+        out.ld[b] = io[b].ldy;
     }
+    launch_spmm_stage(in, L_in, k, kp, xscale, x_f32, wp[0], s);
     if (sb)
         launch_spmm_lines(sb->indexed(adjoint), wp[0], K, x_f32, static_cast<double*>(wp[1]), s);
     else
         launch_dense_block(db->d, adjoint, kp, x_f32, wp[0], static_cast<double*>(wp[1]), static_cast<double*>(wp[5]),
                            dscratch, s);
-    for (int b = 0; b < nio; ++b)
-        launch_spmm_finish(static_cast<const double*>(wp[1]) + b * k, L_out, k, kp, oscale, a.n_terms, tvec[0],
-                           tvec[1], gam + 2 * b * k, io[b].Y, io[b].ldy, s);
+    launch_spmm_finish(static_cast<const double*>(wp[1]), L_out, k, kp, oscale, a.n_terms, tvec[0], tvec[1], gam, out,
+                       s);
     if (adjoint) ctx->stats.adjoint_products += K; else ctx->stats.forward_products += K;
 }
 

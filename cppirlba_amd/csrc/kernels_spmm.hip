@@ -126,21 +126,33 @@ void spmm_dispatch(const TiledLines& t, const void* X, int kp, int lpe, double* 
     }
 }
 
-// X_rm[r * kp + v] = xscale[r] * X[r + v * ld] for v < k, 0 for k <= v < kw
-// (`out` points at the first of the kw packed columns this call writes, so
-// several inputs can share one packed block).  Consecutive threads write
-// consecutive packed values.
+// Entry b of a block list, chosen by comparisons (b varies across lanes, and
+// the lists are kernel arguments).
+template <typename T>
+__device__ __forceinline__ T pick_block(const T (&a)[kMaxPackBlocks], int b) {
+    static_assert(kMaxPackBlocks == 4, "pick_block covers four blocks");
+    return b == 0 ? a[0] : b == 1 ? a[1] : b == 2 ? a[2] : a[3];
+}
+
+// X_rm[r * kp + v] = xscale[r] * X_b[r + c * ld_b] for v = b k + c inside the
+// blocks, 0 in the padding columns.  Consecutive threads write consecutive
+// packed values.
 template <typename FT>
-__global__ void k_spmm_stage(const double* __restrict__ X, i64 ld, i64 L, int k, int kw, int kp,
-                             const double* __restrict__ xscale, FT* __restrict__ out) {
-    const i64 n = L * kw;
+__global__ void k_spmm_stage(StageBlocks in, i64 L, int k, int kp, const double* __restrict__ xscale,
+                             FT* __restrict__ out) {
+    const int kin = in.n * k;
+    const i64 n = L * kp;
     for (i64 e = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
          e += static_cast<i64>(gridDim.x) * blockDim.x) {
-        const i64 r = e / kw;
-        const int v = static_cast<int>(e - r * kw);
-        double x = v < k ? X[r + static_cast<i64>(v) * ld] : 0.0;
-        if (xscale && v < k) x *= xscale[r];
-        out[r * kp + v] = static_cast<FT>(x);
+        const i64 r = e / kp;
+        const int v = static_cast<int>(e - r * kp);
+        double x = 0.0;
+        if (v < kin) {
+            const int b = v / k, c = v - b * k;
+            x = pick_block(in.X, b)[r + static_cast<i64>(c) * pick_block(in.ld, b)];
+            if (xscale) x *= xscale[r];
+        }
+        out[e] = static_cast<FT>(x);
     }
 }
 
@@ -151,30 +163,32 @@ __global__ void k_spmm_stage(const double* __restrict__ X, i64 ld, i64 L, int k,
 // or 32 from k = 64 on, so a tile takes at most ~33 KB of LDS at k = 128.
 static int finish_rows(int k) { return k < 64 ? 64 : 32; }
 
-// Y[m + v * ldy] = oscale[m] * Y_rm[m * kp + v] - sum_j tvec_j[m] * gam[j * k + v].
+// Y_b[m + c * ld_b] = oscale[m] * Y_rm[m * kp + v] - sum_j tvec_j[m] * gam[(2 b + j) k + c]
+// for packed column v = b k + c of block b.
 __global__ __launch_bounds__(256) void k_spmm_finish(const double* __restrict__ Yrm, i64 n, int k, int kp,
                                                      const double* __restrict__ oscale, int n_terms,
                                                      const double* __restrict__ t0, const double* __restrict__ t1,
-                                                     const double* __restrict__ gam, double* __restrict__ Y,
-                                                     i64 ldy, int rows) {
+                                                     const double* __restrict__ gam, FinishBlocks out, int rows) {
     extern __shared__ double tile[];
-    const int ts = k + 1;
+    const int kin = out.n * k;
+    const int ts = kin + 1;
     for (i64 m0 = static_cast<i64>(blockIdx.x) * rows; m0 < n; m0 += static_cast<i64>(gridDim.x) * rows) {
-        for (int e = threadIdx.x; e < rows * k; e += blockDim.x) {
-            const int rr = e / k, v = e - rr * k;
+        for (int e = threadIdx.x; e < rows * kin; e += blockDim.x) {
+            const int rr = e / kin, v = e - rr * kin;
             const i64 m = m0 + rr;
             if (m < n) tile[rr * ts + v] = Yrm[m * kp + v];
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < rows * k; e += blockDim.x) {
+        for (int e = threadIdx.x; e < rows * kin; e += blockDim.x) {
             const int rr = e % rows, v = e / rows;
             const i64 m = m0 + rr;
             if (m < n) {
+                const int b = v / k, c = v - b * k;
                 double o = tile[rr * ts + v];
                 if (oscale) o *= oscale[m];
-                if (n_terms > 0) o -= (t0 ? t0[m] : 1.0) * gam[v];
-                if (n_terms > 1) o -= (t1 ? t1[m] : 1.0) * gam[k + v];
-                Y[m + static_cast<i64>(v) * ldy] = o;
+                if (n_terms > 0) o -= (t0 ? t0[m] : 1.0) * gam[2 * b * k + c];
+                if (n_terms > 1) o -= (t1 ? t1[m] : 1.0) * gam[2 * b * k + k + c];
+                pick_block(out.Y, b)[m + static_cast<i64>(c) * pick_block(out.ld, b)] = o;
             }
         }
         __syncthreads();
@@ -413,17 +427,22 @@ int spmm_padded_k(int k, bool f32, int* lanes_per_entry) {
     return lpe * vn;
 }
 
-void launch_spmm_stage(const double* X, i64 ld, i64 L, int k, int c0, int kw, int kp, const double* xscale, bool f32,
-                       void* out, hipStream_t s) {
-    if (L <= 0 || kw <= 0) return;
-    if (k > kw || c0 < 0 || c0 + kw > kp) throw std::runtime_error("packed block columns out of range");
-    const i64 grid = std::max<i64>(1, std::min<i64>((L * kw + 255) / 256, 16384));
+static void check_blocks(int n, int k, int kp) {
+    if (n < 1 || n > kMaxPackBlocks || k <= 0 || static_cast<i64>(n) * k > kp)
+        throw std::runtime_error("packed block columns out of range");
+}
+
+void launch_spmm_stage(const StageBlocks& in, i64 L, int k, int kp, const double* xscale, bool f32, void* out,
+                       hipStream_t s) {
+    check_blocks(in.n, k, kp);
+    if (L <= 0) return;
+    const i64 grid = std::max<i64>(1, std::min<i64>((L * kp + 255) / 256, 16384));
     if (f32)
-        hipLaunchKernelGGL(k_spmm_stage<float>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, X, ld, L, k, kw,
-                           kp, xscale, static_cast<float*>(out) + c0);
+        hipLaunchKernelGGL(k_spmm_stage<float>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, in, L, k, kp,
+                           xscale, static_cast<float*>(out));
     else
-        hipLaunchKernelGGL(k_spmm_stage<double>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, X, ld, L, k, kw,
-                           kp, xscale, static_cast<double*>(out) + c0);
+        hipLaunchKernelGGL(k_spmm_stage<double>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, in, L, k, kp,
+                           xscale, static_cast<double*>(out));
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
@@ -438,14 +457,16 @@ void launch_spmm_lines(const TiledLines& t, const void* X, int k, bool x_f32, do
 }
 
 void launch_spmm_finish(const double* Yrm, i64 n, int k, int kp, const double* oscale, int n_terms,
-                        const double* t0, const double* t1, const double* gam, double* Y, i64 ldy, hipStream_t s) {
-    if (n <= 0 || k <= 0) return;
-    if (k > kp) throw std::runtime_error("packed block columns out of range");
-    const int rows = finish_rows(k);
+                        const double* t0, const double* t1, const double* gam, const FinishBlocks& out,
+                        hipStream_t s) {
+    check_blocks(out.n, k, kp);
+    if (n <= 0) return;
+    const int kin = out.n * k;
+    const int rows = finish_rows(kin);
     const i64 grid = std::max<i64>(1, std::min<i64>((n + rows - 1) / rows, 8192));
-    const size_t lds = sizeof(double) * rows * (k + 1);
+    const size_t lds = sizeof(double) * rows * (kin + 1);
     hipLaunchKernelGGL(k_spmm_finish, dim3(static_cast<unsigned>(grid)), dim3(256), lds, s, Yrm, n, k, kp, oscale,
-                       n_terms, t0, t1, gam, Y, ldy, rows);
+                       n_terms, t0, t1, gam, out, rows);
     IRLBA_HIP_LAUNCH_CHECK();
 }
 
