@@ -9,7 +9,10 @@
 // 8 x BF16), accumulate in FP32 (FP32/BF16 storage) or FP64, and write one
 // partial per split; a reduce kernel sums the splits in a fixed order and
 // applies the fused scaling/centring/axpy epilogue in FP64.  A prologue kernel
-// scales the input vector into `xs` and computes the rank-1 dot products.
+// scales the input vector into `xs` and computes the rank-1 dot products.  An
+// unsplit dot product without rank-1 terms applies the epilogue itself (no
+// reduce kernel), and an axpy product of an input without scaling or rank-1
+// terms reads the FP64 input directly (no prologue kernel).
 #include "common.hpp"
 
 namespace irlba_hip {
@@ -75,11 +78,22 @@ __global__ void k_dense_prologue(Prologue pro, i64 n, AT* __restrict__ xs, doubl
     }
 }
 
+// Input element j as the axpy products use it: the prologue's scaled copy
+// `xs`, or, for an input without scaling or rank-1 terms (no prologue), the
+// FP64 input converted in registers (the value the prologue would have stored).
+template <typename AT>
+__device__ __forceinline__ AT input_at(const AT* __restrict__ xs, const double* __restrict__ x64, i64 j) {
+    return x64 ? static_cast<AT>(x64[j]) : xs[j];
+}
+
 // ------------------------------------------------------------------ dot-lines
 // Block: R lines x one segment of SEG elements; 256 threads, N elements each per step.
+// With `fused` (one segment per line, no rank-1 terms) the block applies the
+// epilogue and writes the finished outputs instead of partials.
 template <typename VT, typename AT, int N, int R>
 __global__ __launch_bounds__(256) void k_dot_lines(const VT* __restrict__ A, i64 n_lines, i64 Ln, i64 seg_len,
-                                                   const AT* __restrict__ xs, AT* __restrict__ partial) {
+                                                   const AT* __restrict__ xs, AT* __restrict__ partial,
+                                                   Epilogue epi, int fused) {
     __shared__ AT red[R][4];
     const i64 m0 = static_cast<i64>(blockIdx.x) * R;
     const i64 j0 = static_cast<i64>(blockIdx.y) * seg_len;
@@ -110,7 +124,13 @@ __global__ __launch_bounds__(256) void k_dot_lines(const VT* __restrict__ A, i64
     __syncthreads();
     if (threadIdx.x < R && m0 + threadIdx.x < n_lines) {
         const int r = threadIdx.x;
-        partial[static_cast<i64>(blockIdx.y) * n_lines + m0 + r] = red[r][0] + red[r][1] + red[r][2] + red[r][3];
+        const AT sum = red[r][0] + red[r][1] + red[r][2] + red[r][3];
+        if (fused) {
+            const double gamma[2] = {0, 0};
+            epi.out[m0 + r] = apply_epilogue(epi, m0 + r, static_cast<double>(sum), gamma);
+        } else {
+            partial[static_cast<i64>(blockIdx.y) * n_lines + m0 + r] = sum;
+        }
     }
 }
 
@@ -118,7 +138,8 @@ __global__ __launch_bounds__(256) void k_dot_lines(const VT* __restrict__ A, i64
 // Block: 256 threads x N outputs each; grid.y splits the lines into chunks.
 template <typename VT, typename AT, int N>
 __global__ __launch_bounds__(256) void k_axpy_lines(const VT* __restrict__ A, i64 n_lines, i64 Ln, i64 chunk,
-                                                    const AT* __restrict__ xs, AT* __restrict__ partial) {
+                                                    const AT* __restrict__ xs, const double* __restrict__ x64,
+                                                    AT* __restrict__ partial) {
     const i64 i = (static_cast<i64>(blockIdx.x) * 256 + threadIdx.x) * N;
     const i64 l0 = static_cast<i64>(blockIdx.y) * chunk;
     const i64 l1 = min(n_lines, l0 + chunk);
@@ -134,7 +155,7 @@ __global__ __launch_bounds__(256) void k_axpy_lines(const VT* __restrict__ A, i6
             for (int u = 0; u < 4; ++u) load_vec<VT, AT, N>(A + (l + u) * Ln + i, a[u]);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const AT xv = xs[l + u];
+                const AT xv = input_at(xs, x64, l + u);
 #pragma unroll
                 for (int k = 0; k < N; ++k) acc[k] += a[u][k] * xv;
             }
@@ -142,7 +163,7 @@ __global__ __launch_bounds__(256) void k_axpy_lines(const VT* __restrict__ A, i6
         for (; l < l1; ++l) {
             AT a[N];
             load_vec<VT, AT, N>(A + l * Ln + i, a);
-            const AT xv = xs[l];
+            const AT xv = input_at(xs, x64, l);
 #pragma unroll
             for (int k = 0; k < N; ++k) acc[k] += a[k] * xv;
         }
@@ -151,7 +172,8 @@ __global__ __launch_bounds__(256) void k_axpy_lines(const VT* __restrict__ A, i6
     } else {
         for (int k = 0; k < N && i + k < Ln; ++k) {
             AT s = 0;
-            for (i64 l = l0; l < l1; ++l) s += static_cast<AT>(to_double_any(A[l * Ln + i + k])) * xs[l];
+            for (i64 l = l0; l < l1; ++l)
+                s += static_cast<AT>(to_double_any(A[l * Ln + i + k])) * input_at(xs, x64, l);
             partial[static_cast<i64>(blockIdx.y) * Ln + i + k] = s;
         }
     }
@@ -223,9 +245,19 @@ void dense_matvec_typed(const DenseDesc& d, bool adjoint, const Prologue& pro, c
     const i64 n_in = dot ? Ln : n_lines;
     const i64 n_out = dot ? n_lines : Ln;
 
+    // The axpy orientation of an input without scaling or rank-1 terms needs no
+    // prologue: each block reads one input value per line, so it reads the FP64
+    // input and converts it in registers.  The dot orientation re-reads its
+    // input segment in every block and keeps the prologue's compact copy (read
+    // directly in FP64 there, BF16 A.v of 200000 x 2000 took 0.22 ms instead of
+    // 0.125).  Either way every input read ends before an output is written.
+    const bool direct_x = !dot && pro.xscale == nullptr && pro.n_terms == 0;
+    const double* x64 = direct_x ? pro.x : nullptr;
     const int pro_blocks = static_cast<int>(std::max<i64>(1, std::min<i64>((n_in + 255) / 256, 256)));
-    hipLaunchKernelGGL(k_dense_prologue<AT>, dim3(pro_blocks), dim3(256), 0, s, pro, n_in, xs, gamma_scratch);
-    IRLBA_HIP_LAUNCH_CHECK();
+    if (!direct_x) {
+        hipLaunchKernelGGL(k_dense_prologue<AT>, dim3(pro_blocks), dim3(256), 0, s, pro, n_in, xs, gamma_scratch);
+        IRLBA_HIP_LAUNCH_CHECK();
+    }
 
     constexpr int NV = Vec16<VT>::N;
     int splits = 1;
@@ -247,14 +279,20 @@ void dense_matvec_typed(const DenseDesc& d, bool adjoint, const Prologue& pro, c
         if (vec_ok) seg_len = (seg_len + NV - 1) / NV * NV;
         nseg = (Ln + seg_len - 1) / seg_len;
         if (nseg * n_lines > partial_cap) throw std::runtime_error("dense partial buffer too small");
+        // One segment per line and no rank-1 terms: the product kernel finishes
+        // its outputs itself (no reduce kernel).
+        const int fused = nseg == 1 && epi.n_terms == 0;
         const dim3 g(static_cast<unsigned>(groups), static_cast<unsigned>(nseg));
         if (ev0) IRLBA_HIP_CHECK(hipEventRecord(ev0, s));
         if (vec_ok)
-            hipLaunchKernelGGL((k_dot_lines<VT, AT, NV, R>), g, dim3(256), 0, s, A, n_lines, Ln, seg_len, xs, partial);
+            hipLaunchKernelGGL((k_dot_lines<VT, AT, NV, R>), g, dim3(256), 0, s, A, n_lines, Ln, seg_len, xs, partial,
+                               epi, fused);
         else
-            hipLaunchKernelGGL((k_dot_lines<VT, AT, 1, R>), g, dim3(256), 0, s, A, n_lines, Ln, seg_len, xs, partial);
+            hipLaunchKernelGGL((k_dot_lines<VT, AT, 1, R>), g, dim3(256), 0, s, A, n_lines, Ln, seg_len, xs, partial,
+                               epi, fused);
         IRLBA_HIP_LAUNCH_CHECK();
         if (ev1) IRLBA_HIP_CHECK(hipEventRecord(ev1, s));
+        if (fused) return;
         splits = static_cast<int>(nseg);
     } else {
         const bool vec_ok = (Ln % NV) == 0;
@@ -267,9 +305,11 @@ void dense_matvec_typed(const DenseDesc& d, bool adjoint, const Prologue& pro, c
         const dim3 g(static_cast<unsigned>(xblocks), static_cast<unsigned>(nchunk));
         if (ev0) IRLBA_HIP_CHECK(hipEventRecord(ev0, s));
         if (vec_ok)
-            hipLaunchKernelGGL((k_axpy_lines<VT, AT, NV>), g, dim3(256), 0, s, A, n_lines, Ln, chunk, xs, partial);
+            hipLaunchKernelGGL((k_axpy_lines<VT, AT, NV>), g, dim3(256), 0, s, A, n_lines, Ln, chunk, xs, x64,
+                               partial);
         else
-            hipLaunchKernelGGL((k_axpy_lines<VT, AT, 1>), g, dim3(256), 0, s, A, n_lines, Ln, chunk, xs, partial);
+            hipLaunchKernelGGL((k_axpy_lines<VT, AT, 1>), g, dim3(256), 0, s, A, n_lines, Ln, chunk, xs, x64,
+                               partial);
         IRLBA_HIP_LAUNCH_CHECK();
         if (ev1) IRLBA_HIP_CHECK(hipEventRecord(ev1, s));
         splits = static_cast<int>(nchunk);

@@ -310,6 +310,37 @@ def test_products_overwrite_their_output(kind):
             np.testing.assert_array_equal(out.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("storage", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("order", ["C", "F"])
+def test_dense_product_into_its_own_input(storage, order):
+    # A square dense product whose device output is its own input buffer gives
+    # the product of the input as it was, bit-equal to the product into a
+    # separate buffer: every read of the input ends before an output is written
+    # (the dot orientation reads the prologue's copy while it writes finished
+    # outputs, the axpy one reads the input itself and writes partials).
+    import ctypes
+
+    import torch
+    from cppirlba_amd._lib import check, lib
+
+    n = 3000
+    A = np.asarray(np.random.default_rng(5).standard_normal((n, n)), order=order)
+    H = irlba.HipDenseMatrix(A.astype(np.float32) if storage == "f32" else A, storage=storage)
+    for adjoint in (0, 1):
+        xh = np.random.default_rng(11 + adjoint).standard_normal(n)
+        x = torch.from_numpy(xh).cuda()
+        out = torch.empty(n, dtype=torch.float64, device="cuda")
+        same = x.clone()
+        torch.cuda.synchronize()
+        check(lib().irlba_hip_multiply(H.handle, adjoint, ctypes.c_void_p(x.data_ptr()),
+                                       ctypes.c_void_p(out.data_ptr()), 1))
+        check(lib().irlba_hip_multiply(H.handle, adjoint, ctypes.c_void_p(same.data_ptr()),
+                                       ctypes.c_void_p(same.data_ptr()), 1))
+        np.testing.assert_array_equal(same.cpu().numpy(), out.cpu().numpy())
+        ref = H.multiply(xh, adjoint=bool(adjoint))
+        np.testing.assert_array_equal(out.cpu().numpy(), ref)
+
+
 @pytest.mark.parametrize("shape", [(5, 0), (0, 5)])
 @pytest.mark.parametrize("kind", ["dense-C", "dense-F", "dense-f32", "sparse-csc", "sparse-csr"])
 def test_empty_dimension_products(shape, kind):
