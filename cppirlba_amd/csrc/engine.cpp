@@ -491,7 +491,13 @@ TiledLines build_lines(irlba_hip_context* ctx, const std::int32_t* d_idx, const 
     TiledLines t;
     t.n_major = n_major_out;
     t.n_minor = n_minor;
-    t.tile_len = static_cast<int>(std::max<i64>(1, std::min<i64>(tile_len, std::max<i64>(n_minor, 1))));
+    // As many tiles as the LDS budget requires, of equal width (rounded up to 64
+    // entries, within the budget): a remainder tile of a few entries would hold
+    // one short segment per line, whose bookkeeping costs as much as a full one.
+    const i64 tmax = std::max<i64>(1, std::min<i64>(tile_len, std::max<i64>(n_minor, 1)));
+    const i64 n_tiles = std::max<i64>(1, (n_minor + tmax - 1) / tmax);
+    const i64 even = (std::max<i64>(n_minor, 1) + n_tiles - 1) / n_tiles;
+    t.tile_len = static_cast<int>(std::min<i64>(tmax, (even + 63) / 64 * 64));
     t.n_tiles = static_cast<int>(std::max<i64>(1, (n_minor + t.tile_len - 1) / t.tile_len));
     t.nnz = nnz;
     t.st = st;
@@ -669,8 +675,13 @@ void autotune_lanes(irlba_hip_context* ctx, TiledLines& t, bool acc64, void* par
     (void)hipEventDestroy(e1);
 }
 
+// Largest input tile (entries) of the product kernel: 156 KB of the 160 KB LDS
+// of a CU, the rest holding the value-dictionary table (up to 2 KB) and the
+// kernel's reduction scratch.  With tiles of equal width this beat 128 KB on
+// every measured product (A.v of the 30000 x 1e6 matrix: FP32 0.762 -> 0.747 ms,
+// FP64 1.425 -> 1.406 ms).
 int tile_elems(irlba_hip_context* ctx, bool acc64) {
-    size_t budget = std::min<size_t>(128 * 1024, ctx->lds_per_block > 8192 ? ctx->lds_per_block - 4096 : 4096);
+    size_t budget = std::min<size_t>(156 * 1024, ctx->lds_per_block > 8192 ? ctx->lds_per_block - 4096 : 4096);
     if (const char* e = std::getenv("IRLBA_HIP_TILE_BYTES")) budget = std::min<size_t>(budget, std::atol(e));
     return static_cast<int>(budget / (acc64 ? 8 : 4));
 }

@@ -138,7 +138,7 @@ def _csc_dense(vals, idx, ptr, nr, nc):
 @pytest.mark.parametrize("k", [1, 3, 16, 33, 64, 128])
 @pytest.mark.parametrize("decorated", [False, True])
 def test_multiply_block_vs_numpy(storage, accumulate, k, decorated):
-    # Block products on the tiled layout: 70000 columns span three FP32 tiles of
+    # Block products on the tiled layout: 70000 columns span two FP32 tiles of
     # the row orientation; centring + scaling folded in as for the single product.
     nr, nc = 300, 70000
     vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.02, 3, f32=False)
