@@ -55,6 +55,33 @@ def test_sparse_products_fp64(nr, nc, column_major):
     np.testing.assert_array_equal(A.realize(), dense)
 
 
+@pytest.mark.parametrize("storage,acc,nc", [("f32", "f32", 39937), ("f32", "f32", 3 * 39936 + 5),
+                                            ("f64", "f64", 19969), ("f64", "f64", 3 * 19968 + 1),
+                                            ("f32", "f64", 2 * 19968 + 63)])
+def test_equal_width_tiles_vs_oracle(storage, acc, nc):
+    # Input widths just past a whole number of the largest LDS tiles (39936 FP32 or
+    # 19968 FP64-accumulation entries): the tiles are of equal width, a multiple
+    # of 64 (the last one shorter by less than 64 entries per tile), none a
+    # remainder of a few entries, and the products equal the oracle's.
+    nr = 40
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.05, 23, f32=storage == "f32")
+    A = O.sparse(nr, nc, vals, idx, ptr, True, 1)
+    H = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, storage=storage, accumulate=acc)
+    lay = H.layout(False)
+    tmax = 39936 if acc == "f32" else 19968
+    n_tiles = -(-nc // tmax)
+    assert lay["n_tiles"] == n_tiles
+    assert lay["tile_len"] % 64 == 0 and lay["tile_len"] <= tmax
+    last = nc - (n_tiles - 1) * lay["tile_len"]
+    assert lay["tile_len"] - 64 * n_tiles < last <= lay["tile_len"]
+    v = O.create_random_vector(nc, 3)
+    u = O.create_random_vector(nr, 4)
+    # FP32 values (as stored) with FP64 accumulation in the oracle: 2e-6 for FP32 accumulation.
+    tol = 2e-6 if acc == "f32" else 1e-12
+    assert _rel(H.multiply(v), A.multiply(v)) < tol
+    assert _rel(H.adjoint_multiply(u), A.multiply(u, adjoint=True)) < tol
+
+
 def test_sparse_multitile_both_orientations():
     # 40000 columns exceed one FP64 LDS tile, so A.v runs with several tiles and partials.
     nr, nc = 64, 40000
