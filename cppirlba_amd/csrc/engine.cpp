@@ -499,6 +499,7 @@ TiledLines build_lines(irlba_hip_context* ctx, const std::int32_t* d_idx, const 
     const i64 even = (std::max<i64>(n_minor, 1) + n_tiles - 1) / n_tiles;
     t.tile_len = static_cast<int>(std::min<i64>(tmax, (even + 63) / 64 * 64));
     t.n_tiles = static_cast<int>(std::max<i64>(1, (n_minor + t.tile_len - 1) / t.tile_len));
+    require(t.tile_len <= 65536, "input tiles must hold at most 65536 entries (16-bit in-tile indices)");
     t.nnz = nnz;
     t.st = st;
     const i64 nseg = static_cast<i64>(t.n_tiles) * n_major_out;

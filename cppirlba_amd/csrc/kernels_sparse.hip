@@ -990,7 +990,7 @@ __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
                     delta_indices(dw[u], carry, inc[u], tot[u], q);
                     carry += group_last<LPL>(inc[u]);
                     if constexpr (PAIRS) {
-                        // In-tile indices fit 16 bits (tiles of at most 32768 entries).
+                        // In-tile indices fit 16 bits (tiles hold fewer than 65536 entries).
                         const int lo = static_cast<int>(q[0] | (q[1] << 16)), hi = static_cast<int>(q[2] | (q[3] << 16));
                         const int sa = lane_in >> 1, sbb = LPL / 2 + (lane_in >> 1);
                         const int alo = __shfl(lo, sa, LPL), ahi = __shfl(hi, sa, LPL);
