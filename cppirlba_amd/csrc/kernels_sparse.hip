@@ -750,11 +750,14 @@ __global__ __launch_bounds__(1024) void k_spmv_tiled(SpmvParams<AT> p) {
     // Stage the input tile (scaled) in LDS; accumulate the rank-1 dot products.
     double g0 = 0, g1 = 0;
     if (pro.n_terms == 0)
-        pro.xscale ? stage_tile<AT, true, 0>(pro, base, cnt, xs, g0, g1) : stage_tile<AT, false, 0>(pro, base, cnt, xs, g0, g1);
+        pro.xscale ? stage_tile<AT, true, 0>(pro, base, cnt, xs, g0, g1)
+                   : stage_tile<AT, false, 0>(pro, base, cnt, xs, g0, g1);
     else if (pro.n_terms == 1)
-        pro.xscale ? stage_tile<AT, true, 1>(pro, base, cnt, xs, g0, g1) : stage_tile<AT, false, 1>(pro, base, cnt, xs, g0, g1);
+        pro.xscale ? stage_tile<AT, true, 1>(pro, base, cnt, xs, g0, g1)
+                   : stage_tile<AT, false, 1>(pro, base, cnt, xs, g0, g1);
     else
-        pro.xscale ? stage_tile<AT, true, 2>(pro, base, cnt, xs, g0, g1) : stage_tile<AT, false, 2>(pro, base, cnt, xs, g0, g1);
+        pro.xscale ? stage_tile<AT, true, 2>(pro, base, cnt, xs, g0, g1)
+                   : stage_tile<AT, false, 2>(pro, base, cnt, xs, g0, g1);
     if (pro.n_terms > 0) {
         block_sum2(g0, g1, red);
     } else {
