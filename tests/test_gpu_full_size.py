@@ -56,7 +56,11 @@ def test_sparse_pca_30000x1e6_mixed_and_fp64():
     _say(f"  mixed: converged={mixed.converged}, {mixed.iterations} restarts")
     _say("sparse pca 30000x1e6: FP64 top-51 (the 51st value bounds the gap below the 50th)")
     A = irlba.generate_sparse(nr, nc, 0.03, 42, storage="f64", accumulate="f64")
-    f64 = irlba.pca(A, True, True, k + 1, opts)
+    # On this flat spectrum the restart count to tol depends on the rounding of
+    # the products (648 to 1033 restarts across builds for the top 51), so this
+    # run may take more than the default 1000.
+    opts51 = irlba.Options(extra_work=20, initial=init, convergence_tolerance=tol, max_iterations=2000)
+    f64 = irlba.pca(A, True, True, k + 1, opts51)
     _say(f"  fp64: converged={f64.converged}, {f64.iterations} restarts")
     assert mixed.converged and f64.converged
     for res, n in ((mixed, k), (f64, k + 1)):

@@ -473,6 +473,39 @@ def test_dense_products_beyond_2e31_elements(order):
 
 
 # ---------------------------------------------------------------- decorators
+@pytest.mark.parametrize("storage,acc", [("f64", "f64"), ("f32", "f32"), ("f32", "f64")])
+@pytest.mark.parametrize("column,divide", [(True, True), (False, False)])
+def test_centered_scaled_products_multitile(storage, acc, column, divide):
+    # Decorated products whose input spans several LDS tiles (100000 columns: 3
+    # FP32 or 6 FP64-accumulation tiles in A.v): each block scales the tiles it
+    # stages, the block that starts a tile sums its centring dot products, and
+    # the partial reduce adds the tiles' sums.  Against the oracle's decorators,
+    # in both decorator orders and with two centring terms.
+    nr, nc = 40, 100000
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.05, 29, f32=storage == "f32")
+    H = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, storage=storage, accumulate=acc)
+    assert H.layout(False)["n_tiles"] > 1
+    OM = O.sparse(nr, nc, vals, idx, ptr, True, 1)
+    c = O.create_random_vector(nc, 1)
+    s = np.abs(O.create_random_vector(nc if column else nr, 2)) + 0.5
+    tol = 2e-6 if acc == "f32" else 1e-12
+    x = O.create_random_vector(nc, 3)
+    u = O.create_random_vector(nr, 4)
+    pairs = [(irlba.ScaledMatrix(irlba.CenteredMatrix(H, c), s, column, divide),
+              O.scaled(O.centered(OM, c), s, column, divide)),
+             (irlba.CenteredMatrix(irlba.ScaledMatrix(H, s, column, divide), c),
+              O.centered(O.scaled(OM, s, column, divide), c)),
+             (irlba.CenteredMatrix(irlba.CenteredMatrix(H, c), c * 0.5), O.centered(O.centered(OM, c), c * 0.5))]
+    # A row scaling between two centrings keeps two rank-1 terms (different left vectors).
+    sr = np.abs(O.create_random_vector(nr, 5)) + 0.5
+    pairs.append((irlba.CenteredMatrix(irlba.ScaledMatrix(irlba.CenteredMatrix(H, c), sr, False, True), c * 0.5),
+                  O.centered(O.scaled(O.centered(OM, c), sr, False, True), c * 0.5)))
+    for HD, OD in pairs:
+        for _ in range(2):
+            assert _rel(HD.multiply(x), OD.multiply(x)) < tol
+            assert _rel(HD.adjoint_multiply(u), OD.multiply(u, adjoint=True)) < tol
+
+
 @pytest.mark.parametrize("column", [True, False])
 @pytest.mark.parametrize("divide", [True, False])
 @pytest.mark.parametrize("kind", ["dense", "sparse"])
