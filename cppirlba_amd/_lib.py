@@ -85,6 +85,8 @@ SIGNATURES = {
     "irlba_hip_get_stats": (_i, [_VP, ctypes.POINTER(Stats)]),
     "irlba_hip_context_set_profiling": (_i, [_VP, _i]),
     "irlba_hip_context_stream": (_VP, [_VP]),
+    "irlba_hip_context_wait_stream": (_i, [_VP, _VP]),
+    "irlba_hip_stream_wait_context": (_i, [_VP, _VP]),
     "irlba_hip_sparse_create": (_i, [_VP, _i64, _i64, _i, _VP, _VP, _VP, _i, _i, _i, _i, _PP]),
     "irlba_hip_dense_create": (_i, [_VP, _i64, _i64, _i, _VP, _i, _i, _i, _PP]),
     "irlba_hip_centered_create": (_i, [_VP, _VP, _i, _PP]),
@@ -144,6 +146,9 @@ SIGNATURES.update({
                                     _VP, _VP, _VP, _I32, _i]),
     "irlba_hip_sddmm": (_i, [_VP, _i64, _i64, _VP, _VP, _i, _i64, _VP, _VP, _VP, _i, _i]),
     "irlba_hip_sddmm_noalloc": (_i, [_VP, _i64, _i64, _VP, _VP, _i, _i64, _VP, _VP, _VP, _i]),
+    "irlba_hip_svd_backward_async": (_i, [_VP, _i64, _VP, _VP, _VP, _VP, _VP, _VP, _i, ctypes.c_int32,
+                                          ctypes.c_double, _VP, _VP, _VP, _I32]),
+    "irlba_hip_sddmm_async": (_i, [_VP, _i64, _i64, _VP, _VP, _i, _i64, _VP, _VP, _VP, _i]),
     "irlba_hip_multiply_block": (_i, [_VP, _i, _i64, _VP, _VP, _i]),
     "irlba_hip_sparse_layout": (_i, [_VP, _i, ctypes.POINTER(Layout)]),
     "irlba_hip_sparse_value_dictionary": (_i, [_VP, _i, ctypes.POINTER(ctypes.c_int32)]),

@@ -97,14 +97,17 @@ def build_torch():
 
     tlib = os.path.join(os.path.dirname(torch.__file__), "lib")
     abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    # torch's HIP stream header (the binding asks for torch's current stream)
+    # needs the HIP runtime's declarations; no HIP call is made from here.
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
     cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
-           "-Wall", "-Wno-unused-parameter",
+           "-D__HIP_PLATFORM_AMD__", "-Wall", "-Wno-unused-parameter",
            "-I", INCLUDE, "-I", os.path.join(HERE, "include"),
-           *[f"-I{p}" for p in cpp_extension.include_paths()],
+           *[f"-I{p}" for p in cpp_extension.include_paths()], "-I", os.path.join(rocm, "include"),
            os.path.join(CSRC, "torch_svd.cpp"),
            "-o", TORCH_LIB + ".tmp",
            "-L", LIBDIR, "-lirlba_hip", "-Wl,-rpath,$ORIGIN",
-           "-L", tlib, "-ltorch", "-ltorch_cpu", "-lc10", f"-Wl,-rpath,{tlib}"]
+           "-L", tlib, "-ltorch", "-ltorch_cpu", "-lc10_hip", "-lc10", f"-Wl,-rpath,{tlib}"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"g++ failed on torch_svd.cpp:\n{r.stderr}")

@@ -165,6 +165,14 @@ struct irlba_hip_context {
         if (!copy_stream) IRLBA_HIP_CHECK(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
         if (!copy_done) IRLBA_HIP_CHECK(hipEventCreateWithFlags(&copy_done, hipEventDisableTiming));
     }
+    // An event that orders this stream against a caller's stream (a wait takes
+    // the event's latest record, so one event serves both directions).
+    hipEvent_t order_ev = nullptr;
+    void order(hipStream_t from, hipStream_t to) {
+        if (!order_ev) IRLBA_HIP_CHECK(hipEventCreateWithFlags(&order_ev, hipEventDisableTiming));
+        IRLBA_HIP_CHECK(hipEventRecord(order_ev, from));
+        IRLBA_HIP_CHECK(hipStreamWaitEvent(to, order_ev, 0));
+    }
     // Device address of the fixed scratch of the allocation-free entry point on
     // this context's device (resolved when the context is created).
     void* noalloc_scratch = nullptr;
@@ -1534,9 +1542,12 @@ struct LockstepCg {
     }
 };
 
+// `queued`: device arrays only; returns with the work queued on the context's
+// stream and leaves out the read-back of the singular-value check after the
+// rank-k and projected paths (the caller has checked the values).
 int run_svd_backward(const irlba_hip_matrix* m, i64 k, const double* U, const double* sv, const double* V,
                      const double* w, const double* GU, const double* GV, int complement, int max_cg, double cg_tol,
-                     double* X, double* Y, double* dcenter, bool device_ptrs) {
+                     double* X, double* Y, double* dcenter, bool device_ptrs, bool queued) {
     irlba_hip_context* ctx = m->ctx;
     hipStream_t s = ctx->stream;
     const i64 nr = m->base->nr, nc = m->base->nc, R = 2 * k;
@@ -1745,7 +1756,7 @@ int run_svd_backward(const irlba_hip_matrix* m, i64 k, const double* U, const do
         IRLBA_HIP_CHECK(hipMemcpyAsync(Y, Yo, sizeof(double) * nc * R, hipMemcpyDeviceToHost, s));
         if (dcenter) IRLBA_HIP_CHECK(hipMemcpyAsync(dcenter, dc, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
     }
-    check_values();  // synchronises
+    if (!queued) check_values();  // synchronises
     return iterations;
 }
 
@@ -1807,6 +1818,7 @@ int irlba_hip_context_destroy(irlba_hip_context* ctx) {
             (void)hipStreamDestroy(ctx->copy_stream);
         }
         if (ctx->copy_done) (void)hipEventDestroy(ctx->copy_done);
+        if (ctx->order_ev) (void)hipEventDestroy(ctx->order_ev);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
     });
@@ -1839,6 +1851,22 @@ int irlba_hip_context_set_profiling(irlba_hip_context* ctx, int period) {
 }
 
 void* irlba_hip_context_stream(irlba_hip_context* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int irlba_hip_context_wait_stream(irlba_hip_context* ctx, void* stream) {
+    return guarded([&] {
+        require(ctx != nullptr, "null context");
+        CtxGuard g(ctx);
+        ctx->order(static_cast<hipStream_t>(stream), ctx->stream);
+    });
+}
+
+int irlba_hip_stream_wait_context(irlba_hip_context* ctx, void* stream) {
+    return guarded([&] {
+        require(ctx != nullptr, "null context");
+        CtxGuard g(ctx);
+        ctx->order(ctx->stream, static_cast<hipStream_t>(stream));
+    });
+}
 
 int irlba_hip_sparse_create(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, int input_type, const void* values,
                             const int32_t* indices, const int64_t* pointers, int column_major, int storage_type,
@@ -2293,14 +2321,22 @@ int irlba_hip_generate_sparse(irlba_hip_context* ctx, int64_t nrow, int64_t ncol
     });
 }
 
-int irlba_hip_svd_backward(const irlba_hip_matrix* m, int64_t k, const double* U, const double* D, const double* V,
-                           const double* grad_D, const double* grad_U, const double* grad_V, int complement,
-                           int32_t max_cg_iterations, double cg_tolerance, double* X, double* Y, double* grad_center,
-                           int32_t* cg_iterations, int device_pointers) {
+namespace {
+
+// The backward entry points; `queued` returns with the work queued on the
+// context's stream (device arrays and every gradient given, so that nothing
+// the queued kernels read is freed on return).
+int svd_backward_entry(const irlba_hip_matrix* m, int64_t k, const double* U, const double* D, const double* V,
+                       const double* grad_D, const double* grad_U, const double* grad_V, int complement,
+                       int32_t max_cg_iterations, double cg_tolerance, double* X, double* Y, double* grad_center,
+                       int32_t* cg_iterations, int device_pointers, bool queued) {
     return guarded([&] {
         require(m && U && D && V && X && Y, "null argument");
         require(complement >= 0 && complement <= 2, "complement must be 0, 1 or 2");
         require(max_cg_iterations >= 0 && cg_tolerance >= 0, "invalid CG settings");
+        if (queued)
+            require(device_pointers && grad_D && grad_U && grad_V,
+                    "the queued backward takes device arrays and all three gradients");
         irlba_hip_context* ctx = m->ctx;
         CtxGuard g(ctx);
         const i64 nr = m->base->nr, nc = m->base->nc;
@@ -2321,20 +2357,23 @@ int irlba_hip_svd_backward(const irlba_hip_matrix* m, int64_t k, const double* U
         const double* gu = grad_U ? grad_U : zeros(nr * k, zr, dzr);
         const double* gv = grad_V ? grad_V : zeros(nc * k, zc, dzc);
         const int it = run_svd_backward(m, k, U, D, V, w, gu, gv, complement, max_cg_iterations, cg_tolerance, X, Y,
-                                        grad_center, device_pointers != 0);
-        ctx->sync();
+                                        grad_center, device_pointers != 0, queued);
+        if (!queued) ctx->sync();
         if (cg_iterations) *cg_iterations = it;
     });
 }
 
-int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_t* indices,
-                    const int64_t* pointers, int column_major, int64_t rank, const double* X, const double* Y,
-                    void* out, int out_type, int device_pointers) {
+// The SDDMM entry points; `queued` takes device arrays only and returns with
+// the work queued, without reading back the layout checks of the kernel.
+int sddmm_entry(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_t* indices, const int64_t* pointers,
+                int column_major, int64_t rank, const double* X, const double* Y, void* out, int out_type,
+                int device_pointers, bool queued) {
     return guarded([&] {
         require(ctx && pointers && X && Y, "null argument");
         require(nrow >= 0 && ncol >= 0, "negative dimension");
         require(rank >= 1 && rank <= 128, "rank must lie in [1, 128]");
         require(out_type == IRLBA_HIP_F64 || out_type == IRLBA_HIP_F32, "out_type must be F64 or F32");
+        if (queued) require(device_pointers && indices && out, "the queued SDDMM takes device arrays");
         CtxGuard g(ctx);
         hipStream_t s = ctx->stream;
         const bool f32 = out_type == IRLBA_HIP_F32;
@@ -2396,6 +2435,7 @@ int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const in
         if (nnz != 0)
             launch_sddmm(didx, dptr, n_lines, n_other, column_major ? pY : pX, column_major ? pX : pY,
                          static_cast<int>(rank), f32, dout, flag, s);
+        if (queued) return;
         int hflag = 0;
         IRLBA_HIP_CHECK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
         IRLBA_HIP_CHECK(hipStreamSynchronize(s));
@@ -2407,6 +2447,37 @@ int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const in
             IRLBA_HIP_CHECK(hipStreamSynchronize(s));
         }
     });
+}
+
+}  // namespace
+
+int irlba_hip_svd_backward(const irlba_hip_matrix* m, int64_t k, const double* U, const double* D, const double* V,
+                           const double* grad_D, const double* grad_U, const double* grad_V, int complement,
+                           int32_t max_cg_iterations, double cg_tolerance, double* X, double* Y, double* grad_center,
+                           int32_t* cg_iterations, int device_pointers) {
+    return svd_backward_entry(m, k, U, D, V, grad_D, grad_U, grad_V, complement, max_cg_iterations, cg_tolerance, X,
+                              Y, grad_center, cg_iterations, device_pointers, false);
+}
+
+int irlba_hip_svd_backward_async(const irlba_hip_matrix* m, int64_t k, const double* U, const double* D,
+                                 const double* V, const double* grad_D, const double* grad_U, const double* grad_V,
+                                 int complement, int32_t max_cg_iterations, double cg_tolerance, double* X, double* Y,
+                                 double* grad_center, int32_t* cg_iterations) {
+    return svd_backward_entry(m, k, U, D, V, grad_D, grad_U, grad_V, complement, max_cg_iterations, cg_tolerance, X,
+                              Y, grad_center, cg_iterations, 1, true);
+}
+
+int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_t* indices,
+                    const int64_t* pointers, int column_major, int64_t rank, const double* X, const double* Y,
+                    void* out, int out_type, int device_pointers) {
+    return sddmm_entry(ctx, nrow, ncol, indices, pointers, column_major, rank, X, Y, out, out_type, device_pointers,
+                       false);
+}
+
+int irlba_hip_sddmm_async(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_t* indices,
+                          const int64_t* pointers, int column_major, int64_t rank, const double* X, const double* Y,
+                          void* out, int out_type) {
+    return sddmm_entry(ctx, nrow, ncol, indices, pointers, column_major, rank, X, Y, out, out_type, 1, true);
 }
 
 int irlba_hip_sddmm_noalloc(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_t* indices,

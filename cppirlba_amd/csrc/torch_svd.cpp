@@ -3,6 +3,7 @@
 // torch.ops.irlba_hip.svd_sparse / svd_dense.  Host code only; every
 // computation runs in libirlba_hip.so.
 #include <ATen/ATen.h>
+#include <c10/hip/HIPStream.h>
 #include <torch/csrc/autograd/custom_function.h>
 #include <torch/cuda.h>
 #include <torch/custom_class.h>
@@ -89,6 +90,11 @@ at::TensorOptions f64_on(int dev) {
     return dev >= 0 ? t.device(at::Device(at::kCUDA, static_cast<c10::DeviceIndex>(dev))) : t;
 }
 
+// torch's current stream on `dev` (a hipStream_t), for the engine's stream ordering.
+void* torch_stream(int dev) {
+    return static_cast<void*>(c10::hip::getCurrentHIPStream(static_cast<c10::DeviceIndex>(dev)).stream());
+}
+
 // Column-major (rows x k) storage of a (rows x k) gradient, FP64 on `dev`.
 at::Tensor col_major(const at::Tensor& g, int dev) {
     return g.to(f64_on(dev).device(), at::kDouble).t().contiguous();
@@ -116,6 +122,14 @@ variable_list solve(AutogradContext* ctx, c10::intrusive_ptr<Handles> h, const s
     check(irlba_hip_compute(h->op, k, &opt, Ucm.data_ptr<double>(), Vcm.data_ptr<double>(), S.data_ptr<double>(),
                             &nout, &conv, &iters, dev >= 0));
     TORCH_CHECK(nout == k, "the solver returned fewer triplets than requested");
+    // The backward on device arrays does not read back the engine's check of
+    // the singular values (positive, non-increasing); they are checked here,
+    // where the solve has already waited for the device.
+    const at::Tensor Sh = S.to(at::kCPU);
+    const double* sp = Sh.data_ptr<double>();
+    bool values_ok = true;
+    for (int64_t i = 0; i < k; ++i) values_ok = values_ok && sp[i] > 0 && (i == 0 || sp[i] <= sp[i - 1]);
+    ctx->saved_data["values_ok"] = values_ok;
     ctx->saved_data["handles"] = h;
     ctx->saved_data["k"] = k;
     ctx->saved_data["dev"] = static_cast<int64_t>(dev);
@@ -127,36 +141,74 @@ variable_list solve(AutogradContext* ctx, c10::intrusive_ptr<Handles> h, const s
 
 // Factored dL/dA = X Y' (X: 2k x rows, Y: 2k x cols, both column-major views)
 // and the centre gradient, from the saved state and the incoming gradients.
-// `center_edge` is the centre's position among the tensor inputs (the indices
-// that needs_input_grad() takes).
+// On device arrays the engine's work is stream ordered against torch's: its
+// stream waits for torch's current stream after every conversion and
+// allocation the engine reads or writes (so that no block the caching
+// allocator hands out is still in use by earlier torch work), and torch's
+// stream waits for the engine's before anything reads the results.  The
+// tensors the engine reads are held here until then.
 struct Factored {
     at::Tensor X, Y, dcenter;
+    at::Tensor gU, gS, gV;
+    c10::intrusive_ptr<Handles> h;
+    int dev = -1;
 };
 
-Factored factored_grad(AutogradContext* ctx, const variable_list& g, size_t center_edge) {
+// The torch side: gradient conversions and output allocations.  `center_edge`
+// is the centre's position among the tensor inputs (the indices that
+// needs_input_grad() takes).
+Factored prepare_grad(AutogradContext* ctx, const variable_list& g, size_t center_edge) {
     auto saved = ctx->get_saved_variables();
-    const at::Tensor &Ucm = saved[0], &S = saved[1], &Vcm = saved[2];
-    auto h = ctx->saved_data["handles"].toCustomClass<Handles>();
+    const at::Tensor &Ucm = saved[0], &Vcm = saved[2];
+    Factored f;
+    f.h = ctx->saved_data["handles"].toCustomClass<Handles>();
     const int64_t k = ctx->saved_data["k"].toInt();
-    const int dev = static_cast<int>(ctx->saved_data["dev"].toInt());
-    const int complement = static_cast<int>(ctx->saved_data["complement"].toInt());
+    f.dev = static_cast<int>(ctx->saved_data["dev"].toInt());
+    const int dev = f.dev;
     const bool want_center = ctx->saved_data["center"].toBool() && ctx->needs_input_grad(center_edge);
     const int64_t nr = Ucm.size(1), nc = Vcm.size(1);
-    // The incoming gradients and their conversions are ordered on torch's
-    // stream; one synchronisation after them hands them to the engine's stream.
-    at::Tensor gU = g[0].defined() ? col_major(g[0], dev) : at::Tensor();
-    at::Tensor gS = g[1].defined() ? g[1].to(f64_on(dev).device(), at::kDouble).contiguous() : at::Tensor();
-    at::Tensor gV = g[2].defined() ? col_major(g[2], dev) : at::Tensor();
-    if (dev >= 0) torch::cuda::synchronize(dev);
-    Factored f;
+    // On the device every gradient is given (zeros for an unused output), so
+    // that the engine allocates nothing that its queued work would still read.
+    if (g[0].defined()) f.gU = col_major(g[0], dev);
+    else if (dev >= 0) f.gU = at::zeros({k, nr}, f64_on(dev));
+    if (g[1].defined()) f.gS = g[1].to(f64_on(dev).device(), at::kDouble).contiguous();
+    else if (dev >= 0) f.gS = at::zeros({k}, f64_on(dev));
+    if (g[2].defined()) f.gV = col_major(g[2], dev);
+    else if (dev >= 0) f.gV = at::zeros({k, nc}, f64_on(dev));
     f.X = at::empty({2 * k, nr}, f64_on(dev));
     f.Y = at::empty({2 * k, nc}, f64_on(dev));
     if (want_center) f.dcenter = at::empty({nc}, f64_on(dev));
-    auto p = [](const at::Tensor& t) -> double* { return t.defined() ? t.data_ptr<double>() : nullptr; };
-    int32_t cg = 0;
-    check(irlba_hip_svd_backward(h->op, k, p(Ucm), p(S), p(Vcm), p(gS), p(gU), p(gV), complement, 1000, 1e-12,
-                                 p(f.X), p(f.Y), p(f.dcenter), &cg, dev >= 0));
     return f;
+}
+
+double* ptr_of(const at::Tensor& t) { return t.defined() ? t.data_ptr<double>() : nullptr; }
+
+// The engine side: on the device its stream first waits for torch's (every
+// torch-side step of this backward has been queued by now), then the backward
+// is queued; host arrays run synchronously.
+void run_grad(AutogradContext* ctx, Factored& f) {
+    auto saved = ctx->get_saved_variables();
+    const at::Tensor &Ucm = saved[0], &S = saved[1], &Vcm = saved[2];
+    const int64_t k = ctx->saved_data["k"].toInt();
+    const int complement = static_cast<int>(ctx->saved_data["complement"].toInt());
+    int32_t cg = 0;
+    if (f.dev >= 0) {
+        if (!ctx->saved_data["values_ok"].toBool())
+            throw std::runtime_error("singular values must be positive and non-increasing");
+        check(irlba_hip_context_wait_stream(f.h->ctx, torch_stream(f.dev)));
+        check(irlba_hip_svd_backward_async(f.h->op, k, ptr_of(Ucm), ptr_of(S), ptr_of(Vcm), ptr_of(f.gS),
+                                           ptr_of(f.gU), ptr_of(f.gV), complement, 1000, 1e-12, ptr_of(f.X),
+                                           ptr_of(f.Y), ptr_of(f.dcenter), &cg));
+    } else {
+        check(irlba_hip_svd_backward(f.h->op, k, ptr_of(Ucm), ptr_of(S), ptr_of(Vcm), ptr_of(f.gS), ptr_of(f.gU),
+                                     ptr_of(f.gV), complement, 1000, 1e-12, ptr_of(f.X), ptr_of(f.Y),
+                                     ptr_of(f.dcenter), &cg, 0));
+    }
+}
+
+// torch's current stream waits for the engine's queued work (device arrays).
+void torch_after_engine(const Factored& f) {
+    if (f.dev >= 0) check(irlba_hip_stream_wait_context(f.h->ctx, torch_stream(f.dev)));
 }
 
 class SparseSvd : public torch::autograd::Function<SparseSvd> {
@@ -189,24 +241,31 @@ public:
     }
 
     static variable_list backward(AutogradContext* ctx, variable_list g) {
-        Factored f = factored_grad(ctx, g, 3);
+        Factored f = prepare_grad(ctx, g, 3);
+        // The gradient of the values comes out in their own type (FP32 values:
+        // FP32 factors with FP64 dot products).
+        const auto vt = static_cast<at::ScalarType>(ctx->saved_data["values_dtype"].toInt());
+        const bool f32 = vt == at::kFloat;
+        at::Tensor i = ctx->saved_data["indices"].toTensor(), pp = ctx->saved_data["pointers"].toTensor();
         at::Tensor dvalues;
-        if (ctx->needs_input_grad(0)) {
-            auto h = ctx->saved_data["handles"].toCustomClass<Handles>();
-            const int dev = static_cast<int>(ctx->saved_data["dev"].toInt());
-            at::Tensor i = ctx->saved_data["indices"].toTensor(), pp = ctx->saved_data["pointers"].toTensor();
-            // The gradient comes out in the values' own type (FP32 values: FP32
-            // factors with FP64 dot products).
-            const auto vt = static_cast<at::ScalarType>(ctx->saved_data["values_dtype"].toInt());
-            const bool f32 = vt == at::kFloat;
-            dvalues = at::empty({i.numel()}, f64_on(dev).dtype(f32 ? at::kFloat : at::kDouble));
-            check(irlba_hip_sddmm(h->ctx, ctx->saved_data["nrow"].toInt(), ctx->saved_data["ncol"].toInt(),
-                                  i.data_ptr<int32_t>(), pp.data_ptr<int64_t>(),
-                                  ctx->saved_data["column_major"].toBool(), f.X.size(0), f.X.data_ptr<double>(),
-                                  f.Y.data_ptr<double>(), dvalues.data_ptr(), f32 ? IRLBA_HIP_F32 : IRLBA_HIP_F64,
-                                  dev >= 0));
-            if (dvalues.scalar_type() != vt) dvalues = dvalues.to(vt);
+        if (ctx->needs_input_grad(0))
+            dvalues = at::empty({i.numel()}, f64_on(f.dev).dtype(f32 ? at::kFloat : at::kDouble));
+        run_grad(ctx, f);
+        if (dvalues.defined()) {
+            const int64_t nrow = ctx->saved_data["nrow"].toInt(), ncol = ctx->saved_data["ncol"].toInt();
+            const int cm = ctx->saved_data["column_major"].toBool();
+            const int ot = f32 ? IRLBA_HIP_F32 : IRLBA_HIP_F64;
+            // On the device the pattern is that of the matrix the forward
+            // created from these arrays, which checked it.
+            if (f.dev >= 0)
+                check(irlba_hip_sddmm_async(f.h->ctx, nrow, ncol, i.data_ptr<int32_t>(), pp.data_ptr<int64_t>(), cm,
+                                            f.X.size(0), ptr_of(f.X), ptr_of(f.Y), dvalues.data_ptr(), ot));
+            else
+                check(irlba_hip_sddmm(f.h->ctx, nrow, ncol, i.data_ptr<int32_t>(), pp.data_ptr<int64_t>(), cm,
+                                      f.X.size(0), ptr_of(f.X), ptr_of(f.Y), dvalues.data_ptr(), ot, 0));
         }
+        torch_after_engine(f);
+        if (dvalues.defined() && dvalues.scalar_type() != vt) dvalues = dvalues.to(vt);
         return {dvalues, at::Tensor(), at::Tensor(), at::Tensor(), at::Tensor(), at::Tensor(), f.dcenter,
                 at::Tensor(), at::Tensor()};
     }
@@ -235,7 +294,9 @@ public:
     }
 
     static variable_list backward(AutogradContext* ctx, variable_list g) {
-        Factored f = factored_grad(ctx, g, 1);
+        Factored f = prepare_grad(ctx, g, 1);
+        run_grad(ctx, f);
+        torch_after_engine(f);
         at::Tensor dA;
         if (ctx->needs_input_grad(0))
             dA = at::mm(f.X.t(), f.Y).to(static_cast<at::ScalarType>(ctx->saved_data["matrix_dtype"].toInt()));

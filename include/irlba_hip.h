@@ -96,6 +96,14 @@ int irlba_hip_get_stats(const irlba_hip_context* ctx, irlba_hip_stats* out);
 int irlba_hip_context_set_profiling(irlba_hip_context* ctx, int period);
 /* The stream the context launches on (a hipStream_t), for callers timing it. */
 void* irlba_hip_context_stream(irlba_hip_context* ctx);
+/* Stream ordering without a host wait (no reference counterpart): after
+ * irlba_hip_context_wait_stream, work the context queues waits for the work
+ * queued so far on `stream` (a hipStream_t on the context's device, e.g.
+ * torch's current stream, where device inputs are produced); after
+ * irlba_hip_stream_wait_context, work queued on `stream` waits for the work
+ * queued so far on the context (the outputs of the _async entry points). */
+int irlba_hip_context_wait_stream(irlba_hip_context* ctx, void* stream);
+int irlba_hip_stream_wait_context(irlba_hip_context* ctx, void* stream);
 
 /* ParallelSparseMatrix(nrow, ncol, x, i, p, column_major, num_threads)
  * (include/irlba/Matrix/sparse.hpp:438-472).  `values` has type `input_type`
@@ -252,6 +260,19 @@ int irlba_hip_svd_backward(const irlba_hip_matrix* m, int64_t k, const double* U
                            int32_t max_cg_iterations, double cg_tolerance, double* X, double* Y, double* grad_center,
                            int32_t* cg_iterations, int device_pointers);
 
+/* irlba_hip_svd_backward that returns with its work queued on the context's
+ * stream instead of waiting for it.  No reference counterpart: it is for
+ * callers that order their own stream after it (irlba_hip_stream_wait_context),
+ * such as an autograd backward, so that the GPU does not idle for host round
+ * trips.  Device arrays only, all three gradients given (none NULL).  The
+ * rank-k and projected paths do not read back the check that D is positive and
+ * non-increasing (the caller checks D; other values give inf or NaN, not an
+ * error); the exact path checks it before its CG, which synchronises anyway. */
+int irlba_hip_svd_backward_async(const irlba_hip_matrix* m, int64_t k, const double* U, const double* D,
+                                 const double* V, const double* grad_D, const double* grad_U, const double* grad_V,
+                                 int complement, int32_t max_cg_iterations, double cg_tolerance, double* X, double* Y,
+                                 double* grad_center, int32_t* cg_iterations);
+
 /* out[e] = X[row_e, :] . Y[col_e, :] for every stored entry e of a compressed
  * pattern (indices int32, pointers int64; column_major = CSC), with X nrow x rank
  * and Y ncol x rank column-major FP64: the gradient of the stored values of a
@@ -263,6 +284,16 @@ int irlba_hip_svd_backward(const irlba_hip_matrix* m, int64_t k, const double* U
 int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_t* indices,
                     const int64_t* pointers, int column_major, int64_t rank, const double* X, const double* Y,
                     void* out, int out_type, int device_pointers);
+
+/* irlba_hip_sddmm that returns with its work queued on the context's stream
+ * (no reference counterpart; for the same callers as
+ * irlba_hip_svd_backward_async).  Device arrays only, none NULL.  The kernel's
+ * pattern checks are not read back, so a bad pattern is not reported (its
+ * entries are left unwritten or 0): for a pattern already checked, such as that
+ * of a matrix created from the same arrays. */
+int irlba_hip_sddmm_async(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_t* indices,
+                          const int64_t* pointers, int column_major, int64_t rank, const double* X, const double* Y,
+                          void* out, int out_type);
 
 /* irlba_hip_sddmm without any device allocation.  No reference counterpart:
  * it exists for callers that cannot allocate device memory (code under stream

@@ -251,6 +251,98 @@ def test_sddmm_device_pointers_and_checked_layout():
         dev_sddmm(idx, bad_ptr)
 
 
+@pytest.mark.parametrize("complement", ["none", "rank_k", "exact"])
+def test_async_backward_and_sddmm_match_synchronous(complement):
+    # The _async entry points queue the same kernels without waiting for them;
+    # ordered against torch's current stream by an event each way, their
+    # outputs equal the synchronous entry points' bit for bit.  The inputs are
+    # written on torch's stream behind a long matrix product, so a read that
+    # did not wait for that stream would see them unwritten.
+    import ctypes
+
+    import scipy.sparse as sp
+    import torch
+
+    from cppirlba_amd import _lib
+
+    L = _lib.lib()
+    ctx = irlba.default_context()
+    nr, nc, k = 500, 300, 6
+    vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.05, 41, f32=False)
+    A = _dense_of(vals, idx, ptr, nr, nc)
+    h = irlba.HipSparseMatrix(nr, nc, vals, idx, ptr, storage="f64")
+    U, s, V = _triplets(A, k)
+    w, GU, GV = _grads(nr, nc, k, 42)
+
+    def dev(a):  # column-major FP64 on the device
+        a = np.asarray(a, dtype=np.float64)
+        return torch.tensor(a.T.copy().reshape(-1) if a.ndim == 2 else a, device="cuda")
+
+    def vp(t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    ins = [dev(a) for a in (U, s, V, w, GU, GV)]
+    X0 = torch.zeros(nr * 2 * k, dtype=torch.float64, device="cuda")
+    Y0 = torch.zeros(nc * 2 * k, dtype=torch.float64, device="cuda")
+    c0 = torch.zeros(nc, dtype=torch.float64, device="cuda")
+    it0, it1 = ctypes.c_int32(), ctypes.c_int32()
+    code = {"none": 0, "rank_k": 1, "exact": 2}[complement]
+    torch.cuda.synchronize()
+    _lib.check(L.irlba_hip_svd_backward(h.handle, k, vp(ins[0]), vp(ins[1]), vp(ins[2]), vp(ins[3]), vp(ins[4]),
+                                        vp(ins[5]), code, 1000, 1e-12, vp(X0), vp(Y0), vp(c0), ctypes.byref(it0), 1))
+    patterns = {"csc": (idx, ptr, 1)}
+    csr = sp.csr_matrix(A)
+    patterns["csr"] = (csr.indices.astype(np.int32), csr.indptr.astype(np.int64), 0)
+    sync_out = {}
+    for name, (pi, pp, cm) in patterns.items():
+        ti = torch.tensor(pi, device="cuda")
+        tp = torch.tensor(pp, device="cuda")
+        o = torch.zeros(len(pi), dtype=torch.float64, device="cuda")
+        _lib.check(L.irlba_hip_sddmm(ctx.handle, nr, nc, vp(ti), vp(tp), cm, 2 * k, vp(X0), vp(Y0), vp(o),
+                                     _lib.F64, 1))
+        sync_out[name] = o.cpu()
+
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    # Host-to-device copies wait for torch's stream, so they come first.
+    pats = {name: (torch.tensor(pi, device="cuda"), torch.tensor(pp, device="cuda"), cm)
+            for name, (pi, pp, cm) in patterns.items()}
+    big = torch.randn(4096, 4096, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    slow = big @ big  # queued ahead of the input writes on torch's stream
+    late = [torch.empty_like(t) for t in ins]
+    for dst, src in zip(late, ins):
+        dst.copy_(src)
+    X1 = torch.full_like(X0, float("nan"))
+    Y1 = torch.full_like(Y0, float("nan"))
+    c1 = torch.full_like(c0, float("nan"))
+    outs = {name: torch.full((len(pi),), float("nan"), dtype=torch.float64, device="cuda")
+            for name, (pi, _, _) in patterns.items()}
+    _lib.check(L.irlba_hip_context_wait_stream(ctx.handle, stream))
+    _lib.check(L.irlba_hip_svd_backward_async(h.handle, k, vp(late[0]), vp(late[1]), vp(late[2]), vp(late[3]),
+                                              vp(late[4]), vp(late[5]), code, 1000, 1e-12, vp(X1), vp(Y1), vp(c1),
+                                              ctypes.byref(it1)))
+    for name, (ti, tp, cm) in pats.items():
+        _lib.check(L.irlba_hip_sddmm_async(ctx.handle, nr, nc, vp(ti), vp(tp), cm, 2 * k, vp(X1), vp(Y1),
+                                           vp(outs[name]), _lib.F64))
+    _lib.check(L.irlba_hip_stream_wait_context(ctx.handle, stream))
+    # Read on torch's stream, which now waits for the engine's.
+    assert torch.equal(X1.cpu(), X0.cpu()) and torch.equal(Y1.cpu(), Y0.cpu()) and torch.equal(c1.cpu(), c0.cpu())
+    for name in patterns:
+        assert torch.equal(outs[name].cpu(), sync_out[name]), name
+    assert it1.value == it0.value
+    del slow
+    # Device arrays and all three gradients only.
+    with pytest.raises(irlba.HipError, match="all three gradients"):
+        _lib.check(L.irlba_hip_svd_backward_async(h.handle, k, vp(ins[0]), vp(ins[1]), vp(ins[2]), None,
+                                                  vp(ins[4]), vp(ins[5]), code, 1000, 1e-12, vp(X1), vp(Y1), None,
+                                                  None))
+    ti, tp, cm = pats["csc"]
+    with pytest.raises(irlba.HipError, match="device arrays"):
+        _lib.check(L.irlba_hip_sddmm_async(ctx.handle, nr, nc, vp(ti), vp(tp), cm, 2 * k, vp(X1), vp(Y1), None,
+                                           _lib.F64))
+    ctx.synchronize()
+
+
 def test_centered_backward_and_center_gradient():
     nr, nc, k = 300, 120, 5
     vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.1, 21, f32=False)
