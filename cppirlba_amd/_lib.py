@@ -149,6 +149,8 @@ SIGNATURES.update({
     "irlba_hip_svd_backward_async": (_i, [_VP, _i64, _VP, _VP, _VP, _VP, _VP, _VP, _i, ctypes.c_int32,
                                           ctypes.c_double, _VP, _VP, _VP, _I32]),
     "irlba_hip_sddmm_async": (_i, [_VP, _i64, _i64, _VP, _VP, _i, _i64, _VP, _VP, _VP, _i]),
+    "irlba_hip_dense_gradient": (_i, [_VP, _i64, _i64, _i64, _VP, _VP, _VP, _i, _i, _i]),
+    "irlba_hip_dense_gradient_async": (_i, [_VP, _i64, _i64, _i64, _VP, _VP, _VP, _i, _i]),
     "irlba_hip_multiply_block": (_i, [_VP, _i, _i64, _VP, _VP, _i]),
     "irlba_hip_sparse_layout": (_i, [_VP, _i, ctypes.POINTER(Layout)]),
     "irlba_hip_sparse_value_dictionary": (_i, [_VP, _i, ctypes.POINTER(ctypes.c_int32)]),

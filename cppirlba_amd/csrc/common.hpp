@@ -295,6 +295,10 @@ void launch_pack_rows(const double* X, i64 rows, i64 ld, int R, int Rp, bool f32
 // the layout as it reads it (bit 0 of *flag: pointers, bit 1: indices).
 void launch_sddmm(const std::int32_t* idx, const i64* ptr, i64 n_lines, i64 n_other, const void* line_rm,
                   const void* entry_rm, int R, bool f32, void* out, int* flag, hipStream_t s);
+// out = X Y' (X rows x R, Y cols x R, column-major FP64) as a dense rows x cols
+// matrix of type out_t, row-major or column-major.
+void launch_dense_gradient(const double* X, i64 ldx, const double* Y, i64 ldy, int R, i64 rows, i64 cols,
+                           StorageType out_t, bool column_major, void* out, hipStream_t s);
 int gram_blocks(i64 L);
 // out[i * ld_out + j] = A[:, i] . B[:, j] for L x k operands (k <= 64); partials
 // holds gram_blocks(L) * k * k doubles.

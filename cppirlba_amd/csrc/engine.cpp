@@ -2449,6 +2449,49 @@ int sddmm_entry(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_
     });
 }
 
+// The dense-gradient entry points (X Y' as a dense matrix of the output
+// type); `queued` takes device arrays and returns with the work queued.
+int dense_gradient_entry(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, int64_t rank, const double* X,
+                         const double* Y, void* out, int out_type, int column_major, int device_pointers,
+                         bool queued) {
+    return guarded([&] {
+        require(ctx && X && Y && out, "null argument");
+        require(nrow >= 0 && ncol >= 0, "negative dimension");
+        require(rank >= 1 && rank <= 128, "rank must lie in [1, 128]");
+        const StorageType st = to_storage(out_type);
+        if (queued) require(device_pointers, "the queued dense gradient takes device arrays");
+        CtxGuard g(ctx);
+        hipStream_t s = ctx->stream;
+        const i64 n_out = nrow * ncol;
+        const size_t ob = storage_bytes(st);
+        const double *dX = X, *dY = Y;
+        void* dout = out;
+        if (!device_pointers) {
+            auto plan = [&](Carve& c, std::vector<void*>& p) {
+                p.clear();
+                p.push_back(c.take<double>(nrow * rank));
+                p.push_back(c.take<double>(ncol * rank));
+                p.push_back(c.take<char>(static_cast<i64>(ob) * n_out));
+            };
+            std::vector<void*> wp;
+            Carve sizing;
+            plan(sizing, wp);
+            Carve carve;
+            carve.base = ctx->workspace(sizing.off);
+            plan(carve, wp);
+            IRLBA_HIP_CHECK(hipMemcpyAsync(wp[0], X, sizeof(double) * nrow * rank, hipMemcpyHostToDevice, s));
+            IRLBA_HIP_CHECK(hipMemcpyAsync(wp[1], Y, sizeof(double) * ncol * rank, hipMemcpyHostToDevice, s));
+            dX = static_cast<const double*>(wp[0]);
+            dY = static_cast<const double*>(wp[1]);
+            dout = wp[2];
+        }
+        launch_dense_gradient(dX, nrow, dY, ncol, static_cast<int>(rank), nrow, ncol, st, column_major != 0, dout, s);
+        if (!device_pointers && n_out > 0)
+            IRLBA_HIP_CHECK(hipMemcpyAsync(out, dout, ob * n_out, hipMemcpyDeviceToHost, s));
+        if (!queued) ctx->sync();
+    });
+}
+
 }  // namespace
 
 int irlba_hip_svd_backward(const irlba_hip_matrix* m, int64_t k, const double* U, const double* D, const double* V,
@@ -2472,6 +2515,16 @@ int irlba_hip_sddmm(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const in
                     void* out, int out_type, int device_pointers) {
     return sddmm_entry(ctx, nrow, ncol, indices, pointers, column_major, rank, X, Y, out, out_type, device_pointers,
                        false);
+}
+
+int irlba_hip_dense_gradient(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, int64_t rank, const double* X,
+                             const double* Y, void* out, int out_type, int column_major, int device_pointers) {
+    return dense_gradient_entry(ctx, nrow, ncol, rank, X, Y, out, out_type, column_major, device_pointers, false);
+}
+
+int irlba_hip_dense_gradient_async(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, int64_t rank,
+                                   const double* X, const double* Y, void* out, int out_type, int column_major) {
+    return dense_gradient_entry(ctx, nrow, ncol, rank, X, Y, out, out_type, column_major, 1, true);
 }
 
 int irlba_hip_sddmm_async(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, const int32_t* indices,

@@ -174,6 +174,110 @@ void launch_sddmm(const std::int32_t* idx, const i64* ptr, i64 n_lines, i64 n_ot
     }
 }
 
+// ------------------------------------------------------------------ dense gradient
+// out = X Y' written as a dense rows x cols matrix in the matrix's own type,
+// row-major (out[i * cols + j]): the dense counterpart of the SDDMM, so that a
+// dense matrix's gradient needs no FP64 intermediate and no separate
+// conversion.  64 x 64 output tiles, rank slices of 32 staged in LDS, 4 x 4
+// FP64 sums per thread (rows ty + 16u, columns tx + 16v, so a store writes 16
+// consecutive columns of 4 rows); every sum runs over r in increasing order.
+template <typename OT>
+__device__ __forceinline__ OT from_double(double v);
+template <>
+__device__ __forceinline__ double from_double<double>(double v) {
+    return v;
+}
+template <>
+__device__ __forceinline__ float from_double<float>(double v) {
+    return static_cast<float>(v);
+}
+template <>
+__device__ __forceinline__ __hip_bfloat16 from_double<__hip_bfloat16>(double v) {
+    return __float2bfloat16(static_cast<float>(v));  // through FP32, as torch converts
+}
+
+constexpr int kGradTile = 64, kGradSlice = 32;
+
+template <typename OT>
+__global__ __launch_bounds__(256) void k_dense_gradient(const double* __restrict__ X, i64 ldx,
+                                                        const double* __restrict__ Y, i64 ldy, int R, i64 rows,
+                                                        i64 cols, i64 tiles_j, OT* __restrict__ out) {
+    __shared__ double xs[kGradSlice][kGradTile];
+    __shared__ double ys[kGradSlice][kGradTile];
+    const i64 b = blockIdx.x;
+    const i64 i0 = (b / tiles_j) * kGradTile, j0 = (b % tiles_j) * kGradTile;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = 0;
+    for (int r0 = 0; r0 < R; r0 += kGradSlice) {
+        const int rc = min(kGradSlice, R - r0);
+        // Factor columns are contiguous: consecutive threads read consecutive rows.
+        for (int e = threadIdx.x; e < kGradSlice * kGradTile; e += 256) {
+            const int r = e / kGradTile, c = e % kGradTile;
+            const bool live = r < rc;
+            xs[r][c] = live && i0 + c < rows ? X[i0 + c + static_cast<i64>(r0 + r) * ldx] : 0.0;
+            ys[r][c] = live && j0 + c < cols ? Y[j0 + c + static_cast<i64>(r0 + r) * ldy] : 0.0;
+        }
+        __syncthreads();
+        for (int r = 0; r < rc; ++r) {
+            double a[4], bb[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a[u] = xs[r][ty + 16 * u];
+                bb[u] = ys[r][tx + 16 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = __builtin_fma(a[u], bb[v], acc[u][v]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const i64 i = i0 + ty + 16 * u;
+        if (i >= rows) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const i64 j = j0 + tx + 16 * v;
+            if (j < cols) out[i * cols + j] = from_double<OT>(acc[u][v]);
+        }
+    }
+}
+
+void launch_dense_gradient(const double* X, i64 ldx, const double* Y, i64 ldy, int R, i64 rows, i64 cols,
+                           StorageType out_t, bool column_major, void* out, hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return;
+    // A column-major rows x cols output is the row-major output of Y X'.
+    if (column_major) {
+        std::swap(X, Y);
+        std::swap(ldx, ldy);
+        std::swap(rows, cols);
+    }
+    const i64 tiles_j = (cols + kGradTile - 1) / kGradTile;
+    const i64 grid = ((rows + kGradTile - 1) / kGradTile) * tiles_j;
+    if (grid > 0x7fffffff) throw std::runtime_error("dense gradient too large for one launch");
+    const dim3 g(static_cast<unsigned>(grid)), blk(256);
+    switch (out_t) {
+        case ST_F64:
+            hipLaunchKernelGGL(k_dense_gradient<double>, g, blk, 0, s, X, ldx, Y, ldy, R, rows, cols, tiles_j,
+                               static_cast<double*>(out));
+            break;
+        case ST_F32:
+            hipLaunchKernelGGL(k_dense_gradient<float>, g, blk, 0, s, X, ldx, Y, ldy, R, rows, cols, tiles_j,
+                               static_cast<float*>(out));
+            break;
+        default:
+            hipLaunchKernelGGL(k_dense_gradient<__hip_bfloat16>, g, blk, 0, s, X, ldx, Y, ldy, R, rows, cols,
+                               tiles_j, static_cast<__hip_bfloat16*>(out));
+            break;
+    }
+    IRLBA_HIP_LAUNCH_CHECK();
+}
+
 // ------------------------------------------------------------------ Gram products A'B
 constexpr int kGramRows = 32;
 constexpr int kGramMaxK = 64;

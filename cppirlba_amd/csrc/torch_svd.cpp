@@ -295,11 +295,18 @@ public:
 
     static variable_list backward(AutogradContext* ctx, variable_list g) {
         Factored f = prepare_grad(ctx, g, 1);
-        run_grad(ctx, f);
-        torch_after_engine(f);
+        const auto mt = static_cast<at::ScalarType>(ctx->saved_data["matrix_dtype"].toInt());
+        const int64_t nr = f.X.size(1), nc = f.Y.size(1);
+        // On the device the engine writes X Y' in the matrix's type, queued
+        // behind the backward (row-major, the layout of a contiguous matrix).
         at::Tensor dA;
-        if (ctx->needs_input_grad(0))
-            dA = at::mm(f.X.t(), f.Y).to(static_cast<at::ScalarType>(ctx->saved_data["matrix_dtype"].toInt()));
+        if (ctx->needs_input_grad(0) && f.dev >= 0) dA = at::empty({nr, nc}, f64_on(f.dev).dtype(mt));
+        run_grad(ctx, f);
+        if (dA.defined())
+            check(irlba_hip_dense_gradient_async(f.h->ctx, nr, nc, f.X.size(0), ptr_of(f.X), ptr_of(f.Y),
+                                                 dA.data_ptr(), value_type(dA), 0));
+        torch_after_engine(f);
+        if (ctx->needs_input_grad(0) && !dA.defined()) dA = at::mm(f.X.t(), f.Y).to(mt);
         return {dA, f.dcenter, at::Tensor(), at::Tensor()};
     }
 };

@@ -295,6 +295,19 @@ int irlba_hip_sddmm_async(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, co
                           const int64_t* pointers, int column_major, int64_t rank, const double* X, const double* Y,
                           void* out, int out_type);
 
+/* dL/dA = X Y' of irlba_hip_svd_backward written out as a dense nrow x ncol
+ * matrix: the dense counterpart of irlba_hip_sddmm (no reference counterpart).
+ * X nrow x rank and Y ncol x rank column-major FP64; `out_type` F64, F32 or
+ * BF16 (the matrix's own type: every entry is summed in FP64 over the rank in
+ * order, then rounded, BF16 through FP32); out column-major (column_major = 1)
+ * or row-major.  rank <= 128. */
+int irlba_hip_dense_gradient(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, int64_t rank, const double* X,
+                             const double* Y, void* out, int out_type, int column_major, int device_pointers);
+/* The same, returning with the work queued on the context's stream (device
+ * arrays only; for the callers of irlba_hip_svd_backward_async). */
+int irlba_hip_dense_gradient_async(irlba_hip_context* ctx, int64_t nrow, int64_t ncol, int64_t rank,
+                                   const double* X, const double* Y, void* out, int out_type, int column_major);
+
 /* irlba_hip_sddmm without any device allocation.  No reference counterpart:
  * it exists for callers that cannot allocate device memory (code under stream
  * capture, tight training or serving loops).  Same arguments and results as

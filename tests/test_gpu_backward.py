@@ -343,6 +343,58 @@ def test_async_backward_and_sddmm_match_synchronous(complement):
     ctx.synchronize()
 
 
+@pytest.mark.parametrize("nr,nc,R", [(1, 1, 1), (70, 130, 3), (300, 65, 32), (129, 200, 33), (64, 64, 128)])
+@pytest.mark.parametrize("column_major", [True, False])
+def test_dense_gradient_vs_numpy(nr, nc, R, column_major):
+    # X Y' written as a dense matrix in FP64, FP32 and BF16 (ragged tiles, rank
+    # slices of 32 and their remainders), from host and from device arrays; the
+    # queued form gives the same bits.  FP64 to 1e-13 relative; FP32 and BF16
+    # within one rounding (2^-24, 2^-8 relative) of the FP64 product.
+    import ctypes
+
+    import torch
+
+    from cppirlba_amd import _lib
+
+    L = _lib.lib()
+    ctx = irlba.default_context()
+    rng = np.random.default_rng(nr + nc + R)
+    X = np.asfortranarray(rng.standard_normal((nr, R)))
+    Y = np.asfortranarray(rng.standard_normal((nc, R)))
+    ref = X @ Y.T
+    scale = np.abs(X).sum(axis=1)[:, None] * np.abs(Y).sum(axis=1)[None, :]  # bound of each sum's magnitudes
+    order = "F" if column_major else "C"
+    for code, dt, tol in ((_lib.F64, np.float64, 1e-15), (_lib.F32, np.float32, 2.0 ** -24),
+                          (_lib.BF16, None, 2.0 ** -8)):
+        nbytes = 2 if dt is None else np.dtype(dt).itemsize
+        host = np.zeros(nr * nc * nbytes, dtype=np.uint8)
+        _lib.check(L.irlba_hip_dense_gradient(ctx.handle, nr, nc, R, X.ctypes.data, Y.ctypes.data, host.ctypes.data,
+                                              code, int(column_major), 0))
+        if dt is None:  # bfloat16: the high half of an FP32
+            got = (host.view(np.uint16).astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        else:
+            got = host.view(dt).astype(np.float64)
+        got = got.reshape((nr, nc), order=order)
+        err = np.abs(got - ref)
+        # Both FP64 sums (the kernel's, numpy's) lie within R u sum|x_r y_r| of the exact one.
+        assert np.all(err <= 8 * tol * np.abs(ref) + 2.5e-16 * R * scale), (code, err.max())
+        # Device arrays, synchronous and queued.
+        tX = torch.tensor(X.T.copy().reshape(-1), device="cuda")
+        tY = torch.tensor(Y.T.copy().reshape(-1), device="cuda")
+        d0 = torch.zeros(nr * nc * nbytes, dtype=torch.uint8, device="cuda")
+        d1 = torch.full((nr * nc * nbytes,), 255, dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        args = (ctx.handle, nr, nc, R, ctypes.c_void_p(tX.data_ptr()), ctypes.c_void_p(tY.data_ptr()))
+        _lib.check(L.irlba_hip_dense_gradient(*args, ctypes.c_void_p(d0.data_ptr()), code, int(column_major), 1))
+        _lib.check(L.irlba_hip_dense_gradient_async(*args, ctypes.c_void_p(d1.data_ptr()), code,
+                                                    int(column_major)))
+        ctx.synchronize()
+        assert np.array_equal(d0.cpu().numpy(), host) and np.array_equal(d1.cpu().numpy(), host), code
+    with pytest.raises(irlba.HipError, match="rank"):
+        _lib.check(L.irlba_hip_dense_gradient(ctx.handle, nr, nc, 129, X.ctypes.data, Y.ctypes.data,
+                                              host.ctypes.data, _lib.F64, 0, 0))
+
+
 def test_centered_backward_and_center_gradient():
     nr, nc, k = 300, 120, 5
     vals, idx, ptr = O.gen_nb_sparse(nr, nc, 0.1, 21, f32=False)
