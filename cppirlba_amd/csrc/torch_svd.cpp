@@ -206,10 +206,21 @@ void run_grad(AutogradContext* ctx, Factored& f) {
     }
 }
 
-// torch's current stream waits for the engine's queued work (device arrays).
-void torch_after_engine(const Factored& f) {
-    if (f.dev >= 0) check(irlba_hip_stream_wait_context(f.h->ctx, torch_stream(f.dev)));
-}
+// torch's current stream waits for the engine's queued work (device arrays),
+// once: at now(), or when the scope ends if an error unwinds it first, so that
+// no tensor the engine still uses goes back to torch's allocator unordered.
+struct TorchAfterEngine {
+    const Factored& f;
+    bool done = false;
+    explicit TorchAfterEngine(const Factored& fa) : f(fa) {}
+    void now() {
+        done = true;
+        if (f.dev >= 0) check(irlba_hip_stream_wait_context(f.h->ctx, torch_stream(f.dev)));
+    }
+    ~TorchAfterEngine() {
+        if (!done && f.dev >= 0) (void)irlba_hip_stream_wait_context(f.h->ctx, torch_stream(f.dev));
+    }
+};
 
 class SparseSvd : public torch::autograd::Function<SparseSvd> {
 public:
@@ -250,6 +261,7 @@ public:
         at::Tensor dvalues;
         if (ctx->needs_input_grad(0))
             dvalues = at::empty({i.numel()}, f64_on(f.dev).dtype(f32 ? at::kFloat : at::kDouble));
+        TorchAfterEngine order(f);
         run_grad(ctx, f);
         if (dvalues.defined()) {
             const int64_t nrow = ctx->saved_data["nrow"].toInt(), ncol = ctx->saved_data["ncol"].toInt();
@@ -264,7 +276,7 @@ public:
                 check(irlba_hip_sddmm(f.h->ctx, nrow, ncol, i.data_ptr<int32_t>(), pp.data_ptr<int64_t>(), cm,
                                       f.X.size(0), ptr_of(f.X), ptr_of(f.Y), dvalues.data_ptr(), ot, 0));
         }
-        torch_after_engine(f);
+        order.now();
         if (dvalues.defined() && dvalues.scalar_type() != vt) dvalues = dvalues.to(vt);
         return {dvalues, at::Tensor(), at::Tensor(), at::Tensor(), at::Tensor(), at::Tensor(), f.dcenter,
                 at::Tensor(), at::Tensor()};
@@ -301,11 +313,12 @@ public:
         // behind the backward (row-major, the layout of a contiguous matrix).
         at::Tensor dA;
         if (ctx->needs_input_grad(0) && f.dev >= 0) dA = at::empty({nr, nc}, f64_on(f.dev).dtype(mt));
+        TorchAfterEngine order(f);
         run_grad(ctx, f);
         if (dA.defined())
             check(irlba_hip_dense_gradient_async(f.h->ctx, nr, nc, f.X.size(0), ptr_of(f.X), ptr_of(f.Y),
                                                  dA.data_ptr(), value_type(dA), 0));
-        torch_after_engine(f);
+        order.now();
         if (ctx->needs_input_grad(0) && !dA.defined()) dA = at::mm(f.X.t(), f.Y).to(mt);
         return {dA, f.dcenter, at::Tensor(), at::Tensor()};
     }
