@@ -357,7 +357,7 @@ def _dict_leg(mat, res, dt, st, nr, nc):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--mode", choices=["solve", "spmv", "backward", "dense"], default="solve")
-    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--reps", type=int, default=20)  # timed backward passes (median) and spmv products
     ap.add_argument("--storage", default="f32")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
