@@ -156,3 +156,37 @@ def test_sparse_autograd_host_tensors_and_fp32_storage():
         assert x.grad.dtype == dtype
         grads.append(x.grad.double().numpy())
     assert _close(grads[1], grads[0], 1e-4)
+
+
+@pytest.mark.parametrize("complement", ["rank_k", "exact"])
+def test_queued_backward_checks_like_the_engine(complement):
+    # The device backward is queued without reading back the engine's check of
+    # the singular values (positive, non-increasing); the binding makes that
+    # check when the forward returns them.  A rank-1 matrix asked for 3
+    # triplets gives s_2 = s_3 = 0 in the reference algorithm (the engine too
+    # returns exact zeros there, so both backwards fail); whatever values the
+    # forward returns, the queued backward must fail or succeed as the
+    # synchronous C ABI does on the same triplets, with the same message or
+    # the same gradient.
+    rng = np.random.default_rng(77)
+    A0 = np.outer(rng.standard_normal(40), rng.standard_normal(30))
+    A = torch.tensor(A0, device="cuda", requires_grad=True)
+    opts = irlba.Options(extra_work=7)
+    U, S, V = irlba.svd_dense(A, 3, options=opts, complement=complement)
+    w = torch.tensor(rng.standard_normal(3), device="cuda")
+    Ud, Sd, Vd = (t.detach().cpu().numpy() for t in (U, S, V))
+    try:
+        ref = irlba.svd_backward(irlba.HipDenseMatrix(A0), Ud, Sd, Vd, grad_D=w.cpu().numpy(),
+                                 complement=complement)
+        ref_error = None
+    except irlba.HipError as e:
+        ref, ref_error = None, str(e)
+    loss = (w * S).sum()
+    print(f"singular values {Sd}; synchronous backward: {ref_error or 'ran'}")
+    if ref_error is not None:
+        assert "positive and non-increasing" in ref_error
+        with pytest.raises(RuntimeError, match="positive and non-increasing"):
+            loss.backward()
+    else:
+        loss.backward()
+        np.testing.assert_allclose(A.grad.cpu().numpy(), ref.dense(), rtol=1e-10, atol=1e-12)
